@@ -1,0 +1,94 @@
+/*
+ * wloop_gpu.h -- device-level C ABI of libwloop_amd.so (MI355X / gfx950).
+ *
+ * This is the thin layer between the C host code (the module entry points of
+ * smear.h / wloop.h / lnk.h and the ms10 driver) and the hand-written HIP
+ * kernels.  Plain pointers and sizes only; every function returns 0 on
+ * success and a negative code on failure, with the message available from
+ * wlg_error().  The legacy entry points turn a failure into an abort through
+ * error(), as openQCD's error() does in the reference.
+ *
+ * Host link fields handed to this layer use the lexicographic AoS image
+ * double[V][nl][18] (x3 fastest, mu = 0 time), the memory image of
+ * su3_dble per link.  Inside the device a field is stored as planes
+ * (structure of arrays) for coalesced access.
+ */
+#ifndef WLOOP_GPU_H
+#define WLOOP_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wlg_ctx wlg_ctx;
+
+/* Measurement parameters; fields as wloop_parms_t (include/wloop.h) plus the
+ * static-action HYP parameters.  nss[] is cumulative (smear.c:107-120). */
+typedef struct {
+   int msl;
+   int nss[5];
+   int mwlt, mwlr;
+   int op_smear;            /* 0 = APE, 1 = HYP */
+   double alpha_ape;
+   double alpha_hyp[2];
+   int proj_iter;           /* projection enabled iff proj_iter > 0 */
+   double alpha_action[3];  /* HYP of the static action (ms10.c:640) */
+   int hyp_time_only;       /* 0: hyp_links (ms10), 1: hyp_time_links */
+} wlg_parms;
+
+enum { WLG_FP64 = 0, WLG_MIXED = 1 };
+
+/* stage timers, milliseconds accumulated since the last wlg_timers_reset */
+enum {
+   WLG_T_HYP = 0, WLG_T_LEVELS, WLG_T_TIME_TR, WLG_T_SPACE_TR, WLG_T_CONTRACT, WLG_T_FINISH,
+   WLG_T_COUNT
+};
+
+/* Context for one lattice N0 x N1 x N2 x N3 (N0 = time) on one device;
+ * bc = 0 open (U_0 at x0 = N0-1 is zero) or 3 periodic. */
+int wlg_create(wlg_ctx **ctx, const int dims[4], int bc, int device);
+void wlg_destroy(wlg_ctx *ctx);
+const char *wlg_error(void);
+
+/* gauge field transfer (host AoS image [V][4][18]) and synthetic generation */
+int wlg_upload(wlg_ctx *ctx, const double *u);
+int wlg_download(wlg_ctx *ctx, double *u);
+int wlg_generate(wlg_ctx *ctx, uint64_t seed, double eps);
+
+/* HYP smearing of the resident gauge field, in place (hyp_links.c:398-472) */
+int wlg_hyp(wlg_ctx *ctx, const double alpha[3], int proj_iter, int time, int space,
+            int precision);
+/* one APE step of the resident field in place: spatial (four_d = 0,
+ * ape_spatial_links) or all four directions (ape_links), ape_links.c:217-284 */
+int wlg_ape_step(wlg_ctx *ctx, double alpha, int proj_iter, int four_d, int precision);
+/* one 3-d HYP step of the resident field's spatial links (hyp_spatial_links.c) */
+int wlg_hyp_spatial_step(wlg_ctx *ctx, double a1, double a2, int proj_iter, int precision);
+/* v_mu += S_{mu nu}[a,b] on host AoS fields [V][4][18] (staple_sum.c:53-87) */
+int wlg_staple_sum(wlg_ctx *ctx, const double *a, const double *b, int mu, int nu, double *v);
+
+/* Operator levels + transporters + contraction on the resident (already
+ * HYP-smeared) field: wloop_sum (wloop.c:524-688).  wl receives
+ * msl*msl*mwlt*mwlr*N0 doubles in wl_idx order. */
+int wlg_wloop(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
+/* ms10 set_data (ms10.c:633-646): HYP with alpha_action, then wlg_wloop */
+int wlg_measure(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
+/* spatial links of operator level sl of the last wlg_wloop, host AoS [V][3][18] */
+int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
+
+/* device bytes one measurement needs (admission by HBM footprint) */
+size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
+
+int wlg_synchronize(wlg_ctx *ctx);
+void wlg_timers_reset(wlg_ctx *ctx);
+double wlg_timer_ms(wlg_ctx *ctx, int stage);
+/* number of contraction launches and their summed duration (ms), measured with
+ * HIP events on the context's stream */
+int wlg_contract_stats(wlg_ctx *ctx, long *launches, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

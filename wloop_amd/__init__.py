@@ -1,0 +1,198 @@
+"""wloop_amd -- MI355X-native Wilson-loop measurement path (HYP -> APE -> transporters -> W_ij(R,T)).
+
+Python view of the C ABI of ``libwloop_amd.so`` (``include/wloop_gpu.h``).  The
+compute runs in hand-written HIP kernels for gfx950; there is no CPU path: if the
+library is missing, loading raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libwloop_amd.so")
+_lib = None
+
+FP64, MIXED = 0, 1
+APE, HYP = 0, 1
+OPEN, PERIODIC = 0, 3
+T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
+STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
+
+
+class WloopError(RuntimeError):
+    pass
+
+
+class Parms(ctypes.Structure):
+    """wlg_parms of include/wloop_gpu.h."""
+    _fields_ = [("msl", ctypes.c_int), ("nss", ctypes.c_int * 5),
+                ("mwlt", ctypes.c_int), ("mwlr", ctypes.c_int),
+                ("op_smear", ctypes.c_int), ("alpha_ape", ctypes.c_double),
+                ("alpha_hyp", ctypes.c_double * 2), ("proj_iter", ctypes.c_int),
+                ("alpha_action", ctypes.c_double * 3), ("hyp_time_only", ctypes.c_int)]
+
+
+def make_parms(nss, mwlt, mwlr, alpha_ape=0.5, proj_iter=4, alpha_action=(1.0, 1.0, 0.5),
+               op_smear=APE, alpha_hyp=(0.0, 0.0), hyp_time_only=False):
+    """nss is the cumulative per-level step list of the [Wloop] section."""
+    p = Parms()
+    p.msl = len(nss)
+    for i, n in enumerate(nss):
+        p.nss[i] = n
+    p.mwlt, p.mwlr = mwlt, mwlr
+    p.op_smear = op_smear
+    p.alpha_ape = alpha_ape
+    p.alpha_hyp[0], p.alpha_hyp[1] = alpha_hyp
+    p.proj_iter = proj_iter
+    for i in range(3):
+        p.alpha_action[i] = alpha_action[i]
+    p.hyp_time_only = int(hyp_time_only)
+    return p
+
+
+def lib():
+    """Load libwloop_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise WloopError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                         "(make -C wloop_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)
+    L.wlg_create.argtypes = [ctypes.POINTER(vp), ip, ctypes.c_int, ctypes.c_int]
+    L.wlg_destroy.argtypes = [vp]
+    L.wlg_destroy.restype = None
+    L.wlg_error.restype = ctypes.c_char_p
+    L.wlg_upload.argtypes = [vp, dp]
+    L.wlg_download.argtypes = [vp, dp]
+    L.wlg_generate.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
+    L.wlg_hyp.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.wlg_ape_step.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.wlg_hyp_spatial_step.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.wlg_staple_sum.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_int, dp]
+    L.wlg_wloop.argtypes = [vp, ctypes.POINTER(Parms), ctypes.c_int, dp]
+    L.wlg_measure.argtypes = [vp, ctypes.POINTER(Parms), ctypes.c_int, dp]
+    L.wlg_level_download.argtypes = [vp, ctypes.c_int, dp]
+    L.wlg_footprint.argtypes = [ip, ctypes.POINTER(Parms), ctypes.c_int]
+    L.wlg_footprint.restype = ctypes.c_size_t
+    L.wlg_synchronize.argtypes = [vp]
+    L.wlg_timers_reset.argtypes = [vp]
+    L.wlg_timers_reset.restype = None
+    L.wlg_timer_ms.argtypes = [vp, ctypes.c_int]
+    L.wlg_timer_ms.restype = ctypes.c_double
+    L.wlg_contract_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long), dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise WloopError("arrays handed to the library must be C-contiguous float64")
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc):
+    if rc != 0:
+        raise WloopError(lib().wlg_error().decode())
+
+
+def wls(parms, n0):
+    return parms.msl ** 2 * parms.mwlt * parms.mwlr * n0
+
+
+def footprint(dims, parms, precision=FP64):
+    return lib().wlg_footprint((ctypes.c_int * 4)(*dims), ctypes.byref(parms), precision)
+
+
+class Lattice:
+    """One lattice resident on one GPU (wlg_ctx)."""
+
+    def __init__(self, dims, bc=PERIODIC, device=0):
+        self.dims = tuple(int(d) for d in dims)
+        self.bc = bc
+        self.volume = int(np.prod(self.dims))
+        self._h = ctypes.c_void_p()
+        _check(lib().wlg_create(ctypes.byref(self._h), (ctypes.c_int * 4)(*self.dims), bc, device))
+
+    def close(self):
+        if self._h:
+            lib().wlg_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- field transfer -------------------------------------------------
+    def upload(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if u.shape != (self.volume, 4, 18):
+            raise WloopError(f"gauge field must have shape {(self.volume, 4, 18)}, got {u.shape}")
+        _check(lib().wlg_upload(self._h, _dp(u)))
+
+    def download(self):
+        u = np.empty((self.volume, 4, 18))
+        _check(lib().wlg_download(self._h, _dp(u)))
+        return u
+
+    def generate(self, seed, eps):
+        _check(lib().wlg_generate(self._h, seed, eps))
+
+    # --- smearing -------------------------------------------------------
+    def hyp(self, alpha, proj_iter, time=True, space=True, precision=FP64):
+        a = (ctypes.c_double * 3)(*alpha)
+        _check(lib().wlg_hyp(self._h, a, proj_iter, int(time), int(space), precision))
+
+    def ape_step(self, alpha, proj_iter, four_d=False, precision=FP64):
+        _check(lib().wlg_ape_step(self._h, alpha, proj_iter, int(four_d), precision))
+
+    def hyp_spatial_step(self, a1, a2, proj_iter, precision=FP64):
+        _check(lib().wlg_hyp_spatial_step(self._h, a1, a2, proj_iter, precision))
+
+    def staple_sum(self, a, b, mu, nu, v):
+        v = np.ascontiguousarray(v, dtype=np.float64).copy()
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().wlg_staple_sum(self._h, _dp(a), _dp(b), mu, nu, _dp(v)))
+        return v
+
+    # --- measurement ----------------------------------------------------
+    def wloop(self, parms, precision=FP64):
+        out = np.empty(wls(parms, self.dims[0]))
+        _check(lib().wlg_wloop(self._h, ctypes.byref(parms), precision, _dp(out)))
+        return out
+
+    def measure(self, parms, precision=FP64):
+        out = np.empty(wls(parms, self.dims[0]))
+        _check(lib().wlg_measure(self._h, ctypes.byref(parms), precision, _dp(out)))
+        return out
+
+    def level(self, sl):
+        lev = np.empty((self.volume, 3, 18))
+        _check(lib().wlg_level_download(self._h, sl, _dp(lev)))
+        return lev
+
+    def synchronize(self):
+        _check(lib().wlg_synchronize(self._h))
+
+    def timers_reset(self):
+        lib().wlg_timers_reset(self._h)
+
+    def timers(self):
+        return {name: lib().wlg_timer_ms(self._h, i) for i, name in enumerate(STAGES)}
+
+    def contract_stats(self):
+        n = ctypes.c_long()
+        ms = ctypes.c_double()
+        _check(lib().wlg_contract_stats(self._h, ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value

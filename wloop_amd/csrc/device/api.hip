@@ -1,0 +1,583 @@
+// api.hip -- the device-level C ABI (include/wloop_gpu.h): contexts, device
+// memory, streams and the measurement pipeline of ms10 set_data
+// (main/ms10.c:633-646): HYP -> operator levels -> transporters -> contraction.
+//
+// Device memory of a context: the gauge field (4 link slots) plus one scratch
+// arena carved per phase.  HYP uses 24 slots (its level-1 and level-2
+// fields); the measurement uses the operator levels, APE ping-pong slots, the
+// time transporters of every T, two generations of spatial transporters and
+// the per-workgroup partial sums.  A "slot" is one link per site.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <vector>
+
+#include "kernels.h"
+#include "wloop_gpu.h"
+
+using namespace wl;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return -1;
+}
+
+#define CK(call)                                                                          \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+#define CKL()                                                                             \
+  do {                                                                                    \
+    hipError_t e_ = hipGetLastError();                                                    \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__); \
+  } while (0)
+
+struct wlg_ctx {
+  int device;
+  hipStream_t st;
+  Lat L;
+  int bc;
+  double2* U;         // gauge field, 4 slots FP64
+  char* arena;        // scratch
+  size_t arena_bytes;
+  double* wl_dev;
+  size_t wl_bytes;
+  double* partial;
+  size_t partial_bytes;
+  // operator levels of the last measurement (FP64, 3-slot fields)
+  const double2* lev[5];
+  int msl;
+  // timing
+  std::vector<hipEvent_t> ev;
+  size_t ev_used;
+  double t_ms[WLG_T_COUNT];
+  long contract_launches;
+  double contract_ms;
+};
+
+extern "C" const char* wlg_error(void) { return g_err; }
+
+static size_t slot64(const Lat& L) { return (size_t)9 * L.V * sizeof(double2); }
+static size_t slot32(const Lat& L) { return (size_t)9 * L.V * sizeof(float2); }
+
+static int ensure(char** p, size_t* have, size_t need) {
+  if (*have >= need) return 0;
+  if (*p) CK(hipFree(*p));
+  *p = nullptr;
+  *have = 0;
+  CK(hipMalloc((void**)p, need));
+  *have = need;
+  return 0;
+}
+
+static hipEvent_t next_event(wlg_ctx* c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->ev.push_back(e);
+  }
+  return c->ev[c->ev_used++];
+}
+
+static int record(wlg_ctx* c, hipEvent_t* e) {
+  *e = next_event(c);
+  if (!*e) return set_err("hipEventCreate failed");
+  CK(hipEventRecord(*e, c->st));
+  return 0;
+}
+
+extern "C" int wlg_create(wlg_ctx** out, const int dims[4], int bc, int device) {
+  *out = nullptr;
+  for (int m = 0; m < 4; m++)
+    if (dims[m] < 2) return set_err("lattice extent N%d = %d must be >= 2", m, dims[m]);
+  if (bc != 0 && bc != 3)
+    return set_err("only open (0) and periodic (3) boundary conditions are supported");
+  int ndev = 0;
+  CK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return set_err("device %d not present (%d devices)", device, ndev);
+  CK(hipSetDevice(device));
+  wlg_ctx* c = new wlg_ctx();
+  c->device = device;
+  c->bc = bc;
+  for (int m = 0; m < 4; m++) c->L.n[m] = dims[m];
+  c->L.st[3] = 1;
+  c->L.st[2] = dims[3];
+  c->L.st[1] = (long)dims[2] * dims[3];
+  c->L.st[0] = (long)dims[1] * dims[2] * dims[3];
+  c->L.V = (long)dims[0] * c->L.st[0];
+  if (c->L.V > 0x7fffffffL) {
+    delete c;
+    return set_err("lattice volume %ld exceeds 2^31-1 sites", c->L.V);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->U, 4 * slot64(c->L));
+  if (e != hipSuccess) {
+    set_err("context allocation failed: %s", hipGetErrorString(e));
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+extern "C" void wlg_destroy(wlg_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->st);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  if (c->U) (void)hipFree(c->U);
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->wl_dev) (void)hipFree(c->wl_dev);
+  if (c->partial) (void)hipFree(c->partial);
+  (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+extern "C" int wlg_synchronize(wlg_ctx* c) {
+  CK(hipSetDevice(c->device));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+// host AoS [V][nl][18] <-> device planes, staged through the arena
+static int upload_field(wlg_ctx* c, const double* h, int nl, double2* dst) {
+  size_t bytes = (size_t)c->L.V * nl * 18 * sizeof(double);
+  if (ensure(&c->arena, &c->arena_bytes, bytes)) return -1;
+  CK(hipMemcpyAsync(c->arena, h, bytes, hipMemcpyHostToDevice, c->st));
+  launch_aos_to_soa(c->L.V, nl, (const double*)c->arena, dst, c->st);
+  CKL();
+  return 0;
+}
+
+static int download_field(wlg_ctx* c, const double2* src, int nl, double* h) {
+  size_t bytes = (size_t)c->L.V * nl * 18 * sizeof(double);
+  if (ensure(&c->arena, &c->arena_bytes, bytes)) return -1;
+  launch_soa_to_aos(c->L.V, nl, src, (double*)c->arena, c->st);
+  CKL();
+  CK(hipMemcpyAsync(h, c->arena, bytes, hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_upload(wlg_ctx* c, const double* u) {
+  CK(hipSetDevice(c->device));
+  if (upload_field(c, u, 4, c->U)) return -1;
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_download(wlg_ctx* c, double* u) {
+  CK(hipSetDevice(c->device));
+  return download_field(c, c->U, 4, u);
+}
+
+extern "C" int wlg_generate(wlg_ctx* c, uint64_t seed, double eps) {
+  CK(hipSetDevice(c->device));
+  if (ensure(&c->arena, &c->arena_bytes, slot64(c->L))) return -1;
+  launch_generate(c->L, c->U, (double2*)c->arena, seed, eps, c->bc == 0, c->st);
+  CKL();
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int run_hyp(wlg_ctx* c, const double a[3], int piter, int time, int space, int prec) {
+  if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
+  size_t s = slot64(c->L);
+  if (ensure(&c->arena, &c->arena_bytes, 24 * s)) return -1;
+  double2* d1 = (double2*)c->arena;
+  double2* d2 = (double2*)(c->arena + 12 * s);
+  int proj = piter > 0;
+  if (prec == WLG_MIXED)
+    launch_hyp<float, double>(c->L, c->U, d1, d2, a, proj, piter, time, space, c->bc == 0, c->st);
+  else
+    launch_hyp<double, double>(c->L, c->U, d1, d2, a, proj, piter, time, space, c->bc == 0, c->st);
+  CKL();
+  if (c->bc == 0 && !time) {
+    launch_zero_bc(c->L, c->U, c->st);  // set_bc (hyp_links.c:447)
+    CKL();
+  }
+  return 0;
+}
+
+extern "C" int wlg_hyp(wlg_ctx* c, const double alpha[3], int proj_iter, int time, int space,
+                       int prec) {
+  CK(hipSetDevice(c->device));
+  if (run_hyp(c, alpha, proj_iter, time, space, prec)) return -1;
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_ape_step(wlg_ctx* c, double alpha, int proj_iter, int four_d, int prec) {
+  CK(hipSetDevice(c->device));
+  size_t s = slot64(c->L);
+  if (ensure(&c->arena, &c->arena_bytes, 4 * s)) return -1;
+  double2* tmp = (double2*)c->arena;
+  int proj = proj_iter > 0;
+  if (four_d) {
+    launch_ape4(c->L, c->U, tmp, alpha, proj, proj_iter, c->st);
+    CKL();
+    CK(hipMemcpyAsync(c->U, tmp, 4 * s, hipMemcpyDeviceToDevice, c->st));
+  } else {
+    const double2* in = c->U + 9 * c->L.V;
+    if (prec == WLG_MIXED)
+      launch_ape<float, double>(c->L, in, tmp, alpha, proj, proj_iter, c->st);
+    else
+      launch_ape<double, double>(c->L, in, tmp, alpha, proj, proj_iter, c->st);
+    CKL();
+    CK(hipMemcpyAsync(c->U + 9 * c->L.V, tmp, 3 * s, hipMemcpyDeviceToDevice, c->st));
+  }
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_hyp_spatial_step(wlg_ctx* c, double a1, double a2, int proj_iter, int prec) {
+  CK(hipSetDevice(c->device));
+  size_t s = slot64(c->L);
+  if (ensure(&c->arena, &c->arena_bytes, 9 * s)) return -1;
+  double2* D = (double2*)c->arena;
+  double2* out = (double2*)(c->arena + 6 * s);
+  const double2* F = c->U + 9 * c->L.V;
+  if (prec == WLG_MIXED)
+    launch_hyp3<float, double>(c->L, F, D, out, a1, a2, proj_iter > 0, proj_iter, c->st);
+  else
+    launch_hyp3<double, double>(c->L, F, D, out, a1, a2, proj_iter > 0, proj_iter, c->st);
+  CKL();
+  CK(hipMemcpyAsync(c->U + 9 * c->L.V, out, 3 * s, hipMemcpyDeviceToDevice, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_staple_sum(wlg_ctx* c, const double* a, const double* b, int mu, int nu,
+                              double* v) {
+  CK(hipSetDevice(c->device));
+  if (mu < 0 || mu > 3 || nu < 0 || nu > 3 || mu == nu)
+    return set_err("staple_sum: need 0 <= mu,nu <= 3 and mu != nu (got %d,%d)", mu, nu);
+  size_t s = slot64(c->L);
+  size_t aos = (size_t)c->L.V * 4 * 18 * sizeof(double);
+  if (ensure(&c->arena, &c->arena_bytes, 12 * s + aos)) return -1;
+  double2* A = (double2*)c->arena;
+  double2* B = (double2*)(c->arena + 4 * s);
+  double2* Vv = (double2*)(c->arena + 8 * s);
+  char* stage = c->arena + 12 * s;
+  const double* src[3] = {a, b, v};
+  double2* dst[3] = {A, B, Vv};
+  for (int q = 0; q < 3; q++) {
+    CK(hipMemcpyAsync(stage, src[q], aos, hipMemcpyHostToDevice, c->st));
+    launch_aos_to_soa(c->L.V, 4, (const double*)stage, dst[q], c->st);
+    CKL();
+  }
+  launch_staple_sum(c->L, A, B, Vv, mu, nu, c->st);
+  CKL();
+  launch_soa_to_aos(c->L.V, 4, Vv, (double*)stage, c->st);
+  CKL();
+  CK(hipMemcpyAsync(v, stage, aos, hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+static int check_parms(const wlg_ctx* c, const wlg_parms* p) {
+  const Lat& L = c->L;
+  if (p->msl < 1 || p->msl > 5) return set_err("max smearing level out of range (msl=%d)", p->msl);
+  if (p->mwlt < 1 || p->mwlt > L.n[0] / 2 + 1)
+    return set_err("max wloop time extension out of range (mwlt=%d)", p->mwlt);
+  for (int m = 1; m < 4; m++)
+    if (p->mwlr < 1 || p->mwlr > L.n[m] / 2 + 1)
+      return set_err("max wloop spatial extension out of range (mwlr=%d)", p->mwlr);
+  if (p->op_smear != 0 && p->op_smear != 1) return set_err("unknown operator smearing %d", p->op_smear);
+  for (int i = 0; i < p->msl; i++)
+    if (p->nss[i] < 0) return set_err("negative smearing step count nss[%d]=%d", i, p->nss[i]);
+  return 0;
+}
+
+// first T whose time transporter is accumulated in FP64 on the mixed path
+static const int kTimeFp64From = 9;
+
+struct Layout {
+  size_t lev, tmp, lt, lt32, lt64, s, lev32, total;
+};
+
+static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
+  size_t s64 = slot64(L), s32 = slot32(L);
+  Layout o;
+  size_t off = 0;
+  o.lev = off; off += (size_t)3 * p->msl * s64;
+  o.tmp = off; off += (p->op_smear == 1 ? 12 : 6) * s64;
+  if (prec == WLG_MIXED) {
+    o.lev32 = off; off += (size_t)3 * p->msl * s32;
+    o.lt32 = off; off += (size_t)p->mwlt * s32;
+    o.lt64 = off; off += 2 * s64;
+    o.lt = 0;
+    o.s = off; off += (size_t)2 * 3 * p->msl * s32;
+  } else {
+    o.lev32 = o.lt32 = o.lt64 = 0;
+    o.lt = off; off += (size_t)(p->mwlt > 1 ? p->mwlt - 1 : 0) * s64;
+    o.s = off; off += (size_t)2 * 3 * p->msl * s64;
+  }
+  o.total = off;
+  return o;
+}
+
+static size_t partial_count(const Lat& L, const wlg_parms* p) {
+  return (size_t)p->mwlr * p->mwlt * L.n[0] * 3 * contract_chunks(L) * p->msl * p->msl;
+}
+
+extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
+  Lat L;
+  for (int m = 0; m < 4; m++) L.n[m] = dims[m];
+  L.st[3] = 1;
+  L.st[2] = dims[3];
+  L.st[1] = (long)dims[2] * dims[3];
+  L.st[0] = (long)dims[1] * dims[2] * dims[3];
+  L.V = (long)dims[0] * L.st[0];
+  Layout o = wloop_layout(L, p, prec);
+  size_t hyp = 24 * slot64(L);
+  size_t arena = o.total > hyp ? o.total : hyp;
+  size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
+  return 4 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p) * sizeof(double);
+}
+
+template <class TS, class TC>
+static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* const* lev,
+                                  const v2<TS>* const* LT, v2<TS>* sbuf0, v2<TS>* sbuf1,
+                                  int nchunk) {
+  const Lat& L = c->L;
+  const int msl = p->msl;
+  const size_t lvl = (size_t)27 * L.V;  // one 3-slot field in 2-vectors
+  const size_t pstride = (size_t)L.n[0] * 3 * nchunk * msl * msl;
+  const v2<TS>* S[5];
+  const v2<TS>* prev[5];
+  v2<TS>* bufs[2] = {sbuf0, sbuf1};
+  for (int r = 0; r < p->mwlr; r++) {
+    const int R = r + 1;
+    hipEvent_t e0, e1, e2;
+    if (record(c, &e0)) return -1;
+    if (R == 1) {
+      for (int i = 0; i < msl; i++) S[i] = lev[i];
+    } else {
+      v2<TS>* out = bufs[R % 2];
+      for (int i = 0; i < msl; i++) prev[i] = (R == 2) ? lev[i] : bufs[(R - 1) % 2] + i * lvl;
+      launch_space_transporter<TS>(L, msl, lev, prev, out, c->st);
+      CKL();
+      for (int i = 0; i < msl; i++) S[i] = out + i * lvl;
+    }
+    if (record(c, &e1)) return -1;
+    for (int t = 0; t < p->mwlt; t++) {
+      launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S,
+                              c->partial + (size_t)(r * p->mwlt + t) * pstride, nchunk, c->st);
+      CKL();
+    }
+    if (record(c, &e2)) return -1;
+    c->contract_launches += p->mwlt;
+  }
+  return 0;
+}
+
+static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, hipEvent_t e_start) {
+  const Lat& L = c->L;
+  if (check_parms(c, p)) return -1;
+  const int msl = p->msl;
+  Layout o = wloop_layout(L, p, prec);
+  if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
+  size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
+  if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
+  const int nchunk = contract_chunks(L);
+  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p) * sizeof(double)))
+    return -1;
+  const size_t s64 = slot64(L), s32 = slot32(L);
+  const size_t lvl = (size_t)27 * L.V;
+  char* A = c->arena;
+  int proj = p->proj_iter > 0;
+
+  // ---- operator levels (smear.c:99-121): cumulative nss ----
+  const double2* src = c->U + 9 * L.V;  // spatial links of the HYP-smeared field
+  double2* levbuf = (double2*)(A + o.lev);
+  double2* tmp0 = (double2*)(A + o.tmp);
+  double2* tmp1 = tmp0 + lvl;
+  double2* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
+  const double2* lev[5];
+  for (int sl = 0; sl < msl; sl++) {
+    const double2* cur = (sl == 0) ? src : lev[sl - 1];
+    int n = p->nss[sl];
+    if (n == 0) {
+      lev[sl] = cur;
+      continue;
+    }
+    for (int step = 0; step < n; step++) {
+      double2* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
+      if (p->op_smear == 1) {
+        if (prec == WLG_MIXED)
+          launch_hyp3<float, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
+                                     p->proj_iter, c->st);
+        else
+          launch_hyp3<double, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
+                                      p->proj_iter, c->st);
+      } else {
+        if (prec == WLG_MIXED)
+          launch_ape<float, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+        else
+          launch_ape<double, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+      }
+      CKL();
+      cur = dst;
+    }
+    lev[sl] = levbuf + sl * lvl;
+  }
+  for (int sl = 0; sl < msl; sl++) c->lev[sl] = lev[sl];
+  c->msl = msl;
+  hipEvent_t e_lev;
+  if (record(c, &e_lev)) return -1;
+
+  // ---- time transporters L_T for every T (wloop.c:153-204) ----
+  size_t ev_before_rt = 0;
+  hipEvent_t e_tt;
+  if (prec == WLG_MIXED) {
+    float2* lt32 = (float2*)(A + o.lt32);
+    double2* ch[2] = {(double2*)(A + o.lt64), (double2*)(A + o.lt64 + s64)};
+    float2* lev32 = (float2*)(A + o.lev32);
+    const float2* lev32p[5];
+    std::vector<const float2*> lts(p->mwlt);
+    launch_d2f((long)9 * L.V, c->U, lt32, c->st);  // L_1 = U_0 in FP32
+    CKL();
+    lts[0] = lt32;
+    for (int T = 2; T <= p->mwlt; T++) {
+      float2* out32 = lt32 + (size_t)(T - 1) * 9 * L.V;
+      if (T < kTimeFp64From) {
+        launch_time_transporter<float, float, float, float>(L, lt32, 0, lts[T - 2], out32, nullptr,
+                                                            c->st);
+      } else if (T == kTimeFp64From) {
+        launch_time_transporter<double, double, float, double>(L, c->U, 0, lts[T - 2], ch[T % 2],
+                                                               out32, c->st);
+      } else {
+        launch_time_transporter<double, double, double, double>(L, c->U, 0, ch[(T - 1) % 2],
+                                                                ch[T % 2], out32, c->st);
+      }
+      CKL();
+      lts[T - 1] = out32;
+    }
+    for (int sl = 0; sl < msl; sl++) {
+      launch_d2f(lvl, lev[sl], lev32 + sl * lvl, c->st);
+      CKL();
+      lev32p[sl] = lev32 + sl * lvl;
+    }
+    if (record(c, &e_tt)) return -1;
+    ev_before_rt = c->ev_used;
+    float2* sb = (float2*)(A + o.s);
+    if (run_transport_contract<float, float>(c, p, lev32p, lts.data(), sb, sb + 3 * msl * 9 * L.V,
+                                             nchunk))
+      return -1;
+    (void)s32;
+  } else {
+    std::vector<const double2*> lts(p->mwlt);
+    lts[0] = c->U;  // slot 0 of the gauge field
+    double2* ltb = (double2*)(A + o.lt);
+    for (int T = 2; T <= p->mwlt; T++) {
+      double2* out = ltb + (size_t)(T - 2) * 9 * L.V;
+      launch_time_transporter<double, double, double, double>(L, c->U, 0, lts[T - 2], out, nullptr,
+                                                              c->st);
+      CKL();
+      lts[T - 1] = out;
+    }
+    if (record(c, &e_tt)) return -1;
+    ev_before_rt = c->ev_used;
+    double2* sb = (double2*)(A + o.s);
+    if (run_transport_contract<double, double>(c, p, lev, lts.data(), sb, sb + 3 * msl * 9 * L.V,
+                                               nchunk))
+      return -1;
+  }
+  size_t ev_after_rt = c->ev_used;
+
+  // ---- normalisation and wl_idx store (wloop.c:535, :584-597) ----
+  double norm = 9.0 * L.n[1] * L.n[2] * L.n[3];
+  launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, nchunk, c->partial, norm, c->wl_dev, c->st);
+  CKL();
+  hipEvent_t e_end;
+  if (record(c, &e_end)) return -1;
+  if (wl_host) CK(hipMemcpyAsync(wl_host, c->wl_dev, wls * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+
+  // ---- timers ----
+  float ms = 0.f;
+  if (e_start) {
+    CK(hipEventElapsedTime(&ms, e_start, e_lev));
+    c->t_ms[WLG_T_LEVELS] += ms;
+  }
+  CK(hipEventElapsedTime(&ms, e_lev, e_tt));
+  c->t_ms[WLG_T_TIME_TR] += ms;
+  for (size_t q = ev_before_rt; q + 2 < ev_after_rt; q += 3) {
+    float a = 0.f, b = 0.f;
+    CK(hipEventElapsedTime(&a, c->ev[q], c->ev[q + 1]));
+    CK(hipEventElapsedTime(&b, c->ev[q + 1], c->ev[q + 2]));
+    c->t_ms[WLG_T_SPACE_TR] += a;
+    c->t_ms[WLG_T_CONTRACT] += b;
+    c->contract_ms += b;
+  }
+  CK(hipEventElapsedTime(&ms, c->ev[ev_after_rt - 1], e_end));
+  c->t_ms[WLG_T_FINISH] += ms;
+  return 0;
+}
+
+extern "C" int wlg_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl) {
+  CK(hipSetDevice(c->device));
+  c->ev_used = 0;
+  hipEvent_t e0;
+  if (record(c, &e0)) return -1;
+  return run_wloop(c, p, prec, wl, e0);
+}
+
+extern "C" int wlg_measure(wlg_ctx* c, const wlg_parms* p, int prec, double* wl) {
+  CK(hipSetDevice(c->device));
+  if (check_parms(c, p)) return -1;
+  c->ev_used = 0;
+  hipEvent_t e0, e1;
+  if (record(c, &e0)) return -1;
+  int piter = p->proj_iter;
+  if (run_hyp(c, p->alpha_action, piter, 1, p->hyp_time_only ? 0 : 1, prec)) return -1;
+  if (record(c, &e1)) return -1;
+  if (run_wloop(c, p, prec, wl, e1)) return -1;
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, e0, e1));
+  c->t_ms[WLG_T_HYP] += ms;
+  return 0;
+}
+
+extern "C" int wlg_level_download(wlg_ctx* c, int sl, double* lev) {
+  CK(hipSetDevice(c->device));
+  if (sl < 0 || sl >= c->msl) return set_err("level %d not available (msl=%d)", sl, c->msl);
+  size_t bytes = (size_t)c->L.V * 3 * 18 * sizeof(double);
+  double* stage = nullptr;
+  CK(hipMalloc((void**)&stage, bytes));
+  launch_soa_to_aos(c->L.V, 3, c->lev[sl], stage, c->st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(lev, stage, bytes, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  (void)hipFree(stage);
+  if (e != hipSuccess) return set_err("level download failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" void wlg_timers_reset(wlg_ctx* c) {
+  for (int i = 0; i < WLG_T_COUNT; i++) c->t_ms[i] = 0.0;
+  c->contract_ms = 0.0;
+  c->contract_launches = 0;
+}
+
+extern "C" double wlg_timer_ms(wlg_ctx* c, int stage) {
+  return (stage >= 0 && stage < WLG_T_COUNT) ? c->t_ms[stage] : -1.0;
+}
+
+extern "C" int wlg_contract_stats(wlg_ctx* c, long* launches, double* ms) {
+  *launches = c->contract_launches;
+  *ms = c->contract_ms;
+  return 0;
+}

@@ -1,0 +1,46 @@
+// kernels.h -- host-side launchers of the gfx950 kernels (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "su3.hpp"
+
+namespace wl {
+
+// smear.hip
+template <class TC, class TS>
+void launch_hyp(const Lat& L, v2<TS>* U, v2<TS>* D1, v2<TS>* D2, const double a[3], int proj,
+                int piter, int time, int space, int open_bc, hipStream_t st);
+template <class TC, class TS>
+void launch_hyp3(const Lat& L, const v2<TS>* F, v2<TS>* D, v2<TS>* out, double a1, double a2,
+                 int proj, int piter, hipStream_t st);
+template <class TC, class TS>
+void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
+                hipStream_t st);
+void launch_ape4(const Lat& L, const double2* in, double2* out, double alpha, int proj, int piter,
+                 hipStream_t st);
+void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2* v, int mu, int nu,
+                       hipStream_t st);
+
+// gen.hip
+void launch_generate(const Lat& L, double2* U, double2* G, uint64_t seed, double eps, int open_bc,
+                     hipStream_t st);
+void launch_aos_to_soa(long V, int nl, const double* a, double2* f, hipStream_t st);
+void launch_soa_to_aos(long V, int nl, const double2* f, double* a, hipStream_t st);
+void launch_zero_bc(const Lat& L, double2* U, hipStream_t st);
+void launch_d2f(long n, const double2* a, float2* b, hipStream_t st);
+
+// wloop.hip
+template <class TC, class TU, class TP, class TO>
+void launch_time_transporter(const Lat& L, const v2<TU>* U, int lu, const v2<TP>* prev, v2<TO>* out,
+                             float2* out32, hipStream_t st);
+template <class TS>
+void launch_space_transporter(const Lat& L, int msl, const v2<TS>* const* F,
+                              const v2<TS>* const* prev, v2<TS>* out, hipStream_t st);
+int contract_chunks(const Lat& L);
+template <class TS, class TC>
+void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
+                     const v2<TS>* const* S, double* partial, int nchunk, hipStream_t st);
+void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nchunk, const double* partial,
+                         double norm, double* wl, hipStream_t st);
+
+}  // namespace wl

@@ -1,0 +1,291 @@
+// smear.hip -- HYP (4-d, 3 levels), 3-d HYP and APE smearing kernels.
+//
+// One thread computes one output link.  Staples are summed in the compute
+// type TC (double on the plain path, float on the mixed path); the
+// lincomb (1-a)U + b*S and the approximate SU(3) projection always run in
+// FP64.  Link storage type TS is double or float.
+//
+// Reference algorithm: modules/smear/hyp_links.c:154-373 (HYP levels),
+// modules/smear/hyp_spatial_links.c:140-283 (3-d HYP),
+// modules/smear/ape_links.c:147-262 (APE step), staple formula
+// modules/lnk/lnkstaple.c:43-56, :318-382.
+#include "su3.hpp"
+#include "kernels.h"
+
+namespace wl {
+
+// v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
+// ln: slot of the nu-links in A, lm: slot of the mu-links in B.
+template <class TC, class TS>
+__device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, long s, const int x[4],
+                                           const v2<TS>* __restrict__ A, int ln,
+                                           const v2<TS>* __restrict__ B, int lm, int mu, int nu) {
+  M3<TS> a, b, c;
+  long xpn = fwd(L, s, x, nu, 1);
+  long xpm = fwd(L, s, x, mu, 1);
+  load(b, B, L.V, lm, xpn);
+  load(c, A, L.V, ln, xpm);
+  load(a, A, L.V, ln, s);
+  M3<TC> t = mul_dag(convert<TC>(b), convert<TC>(c));
+  t = mul(convert<TC>(a), t);
+  acc(v, t);
+  long xmn = bwd(L, s, x, nu);
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  y[nu] = (x[nu] == 0) ? L.n[nu] - 1 : x[nu] - 1;
+  long xmnpm = fwd(L, xmn, y, mu, 1);
+  load(b, B, L.V, lm, xmn);
+  load(c, A, L.V, ln, xmnpm);
+  load(a, A, L.V, ln, xmn);
+  t = mul(convert<TC>(b), convert<TC>(c));
+  t = dag_mul(convert<TC>(a), t);
+  acc(v, t);
+}
+
+// out = P((1-a)U + b S), S in TC, U and out in TS, combination in FP64
+template <class TC, class TS>
+__device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __restrict__ U, int lu,
+                                               v2<TS>* __restrict__ out, int lo, long V, long s,
+                                               double alpha, double beta, int proj, int piter) {
+  M3<TS> u;
+  load(u, U, V, lu, s);
+  M3<double> w;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    w.re[e] = alpha * (double)u.re[e] + beta * (double)S.re[e];
+    w.im[e] = alpha * (double)u.im[e] + beta * (double)S.im[e];
+  }
+  if (proj) project(w, piter);
+  store(convert<TS>(w), out, V, lo, s);
+}
+
+__device__ __forceinline__ int part4(int a, int b) {
+  // pair partition {0,1},{2,3} -> 0 ; {0,2},{1,3} -> 1 ; {0,3},{1,2} -> 2
+  int d = a + b;
+  if (d > 3) d = 6 - d;
+  return d - 1;
+}
+
+// Level 1: V1[part{mu,eta}]_mu = P((1-a3) U_mu + a3/2 S_{mu eta}[U,U]);
+// blockIdx.y = 3*mu + (i-1), eta = (mu+i)&3.  Output slot part*4+mu.
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restrict__ U,
+                                                  v2<TS>* __restrict__ D1, double alpha,
+                                                  double beta, int proj, int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = blockIdx.y / 3, eta = (mu + 1 + blockIdx.y % 3) & 3;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  staple_add<TC, TS>(S, L, s, x, U, eta, U, mu, mu, eta);
+  decorate_store<TC, TS>(S, U, mu, D1, 4 * part4(mu, eta) + mu, L.V, s, alpha, beta, proj, piter);
+}
+
+// Level 2: V2[part{mu,nu}]_mu = P((1-a2)U_mu + a2/4 sum_rho S_{mu rho}[A,B]),
+// rho-links A = V1[part{mu,nu}], mu-links B = V1[part{nu,rho}], rho ascending.
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restrict__ U,
+                                                  const v2<TS>* __restrict__ D1,
+                                                  v2<TS>* __restrict__ D2, double alpha,
+                                                  double beta, int proj, int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = blockIdx.y / 3, nu = (mu + 1 + blockIdx.y % 3) & 3;
+  int n = part4(mu, nu);
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  for (int rho = 0; rho < 4; rho++) {
+    if (rho == mu || rho == nu) continue;
+    staple_add<TC, TS>(S, L, s, x, D1, 4 * n + rho, D1, 4 * part4(nu, rho) + mu, mu, rho);
+  }
+  decorate_store<TC, TS>(S, U, mu, D2, 4 * n + mu, L.V, s, alpha, beta, proj, piter);
+}
+
+// Level 3: V_mu = P((1-a1)U_mu + a1/6 sum_{i=1..3} S_{mu nu}[V2[part{nu,mu}]]),
+// nu = (mu+i)&3.  Written in place into U (each thread reads only its own
+// U_mu(x) besides D2), blockIdx.y = mu - mu0.
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void hyp_level3(Lat L, v2<TS>* __restrict__ U,
+                                                  const v2<TS>* __restrict__ D2, int mu0,
+                                                  double alpha, double beta, int proj,
+                                                  int piter, int open_bc) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = mu0 + blockIdx.y;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  for (int i = 1; i < 4; i++) {
+    int nu = (mu + i) & 3;
+    int n = part4(nu, mu);
+    staple_add<TC, TS>(S, L, s, x, D2, 4 * n + nu, D2, 4 * n + mu, mu, nu);
+  }
+  decorate_store<TC, TS>(S, U, mu, U, mu, L.V, s, alpha, beta, proj, piter);
+  if (open_bc && mu == 0 && x[0] == L.n[0] - 1) {
+    M3<TS> z;
+    set_zero(z);
+    store(z, U, L.V, 0, s);  // set_bc: U_0(x0 = N0-1) = 0
+  }
+}
+
+// 3-d HYP level 1 on a spatial field F (slots k = mu-1):
+// V1[slot(mu,eta)]_mu = P((1-a2)U + a2/2 S_{mu eta}[F,F]);  slot(mu,eta) as
+// hyp_spatial_links.c:81-86 ({1,2},{2,1},{3,1} -> 0, else 1).
+__device__ __forceinline__ int part3(int a, int b) {
+  return ((a == 1 && b == 2) || (a == 2 && b == 1) || (a == 3 && b == 1)) ? 0 : 1;
+}
+
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void hyp3_level1(Lat L, const v2<TS>* __restrict__ F,
+                                                   v2<TS>* __restrict__ D, double alpha,
+                                                   double beta, int proj, int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = 1 + blockIdx.y / 2, eta = ((mu + blockIdx.y % 2) % 3) + 1;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  staple_add<TC, TS>(S, L, s, x, F, eta - 1, F, mu - 1, mu, eta);
+  decorate_store<TC, TS>(S, F, mu - 1, D, 3 * part3(mu, eta) + mu - 1, L.V, s, alpha, beta, proj,
+                         piter);
+}
+
+// 3-d HYP level 2: nu-links from slot(nu,mu), mu-links from slot(mu,nu),
+// nu = ((mu+i)%3)+1 for i = 0,1 (hyp_spatial_links.c:187-234).
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restrict__ F,
+                                                   const v2<TS>* __restrict__ D,
+                                                   v2<TS>* __restrict__ out, double alpha,
+                                                   double beta, int proj, int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = 1 + blockIdx.y;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  for (int i = 0; i < 2; i++) {
+    int nu = ((mu + i) % 3) + 1;
+    staple_add<TC, TS>(S, L, s, x, D, 3 * part3(nu, mu) + nu - 1, D, 3 * part3(mu, nu) + mu - 1, mu,
+                       nu);
+  }
+  decorate_store<TC, TS>(S, F, mu - 1, out, mu - 1, L.V, s, alpha, beta, proj, piter);
+}
+
+// One APE step on spatial links (ape_links.c:147-262 with time = 0):
+// out_mu = P(in_mu + alpha * sum_{eta != mu, spatial, ascending} S_{mu eta}[in,in]).
+// in and out are 3-slot spatial fields (slot k = mu-1); Jacobi update.
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void ape_step(Lat L, const v2<TS>* __restrict__ in,
+                                                v2<TS>* __restrict__ out, double alpha, int proj,
+                                                int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = 1 + blockIdx.y;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  for (int eta = 1; eta < 4; eta++) {
+    if (eta == mu) continue;
+    staple_add<TC, TS>(S, L, s, x, in, eta - 1, in, mu - 1, mu, eta);
+  }
+  decorate_store<TC, TS>(S, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
+}
+
+// 4-d APE step (ape_links.c with time = 1) on a 4-slot field.
+template <class TC, class TS>
+__global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict__ in,
+                                                 v2<TS>* __restrict__ out, double alpha, int proj,
+                                                 int piter) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = blockIdx.y;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S;
+  set_zero(S);
+  for (int eta = 0; eta < 4; eta++) {
+    if (eta == mu) continue;
+    staple_add<TC, TS>(S, L, s, x, in, eta, in, mu, mu, eta);
+  }
+  decorate_store<TC, TS>(S, in, mu, out, mu, L.V, s, 1.0, alpha, proj, piter);
+}
+
+// Generic staple sum on a 4-slot field: v_mu += S_{mu nu}[A,B] (staple_sum.c:53-87).
+__global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* __restrict__ A,
+                                                         const double2* __restrict__ B,
+                                                         double2* __restrict__ v, int mu, int nu) {
+  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int x[4];
+  coords(L, s, x);
+  M3<double> S;
+  load(S, v, L.V, mu, s);
+  staple_add<double, double>(S, L, s, x, A, nu, B, mu, mu, nu);
+  store(S, v, L.V, mu, s);
+}
+
+// ---- launchers --------------------------------------------------------------
+
+static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)((V + 255) / 256), ny, 1); }
+
+template <class TC, class TS>
+void launch_hyp(const Lat& L, v2<TS>* U, v2<TS>* D1, v2<TS>* D2, const double a[3], int proj,
+                int piter, int time, int space, int open_bc, hipStream_t st) {
+  hipLaunchKernelGGL((hyp_level1<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1,
+                     1.0 - a[2], a[2] / 2.0, proj, piter);
+  hipLaunchKernelGGL((hyp_level2<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1, D2,
+                     1.0 - a[1], a[1] / 4.0, proj, piter);
+  int mu0 = 1 - (time > 0), mu1 = 1 + 3 * (space > 0);
+  if (mu1 > mu0)
+    hipLaunchKernelGGL((hyp_level3<TC, TS>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U, D2,
+                       mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
+}
+
+template <class TC, class TS>
+void launch_hyp3(const Lat& L, const v2<TS>* F, v2<TS>* D, v2<TS>* out, double a1, double a2,
+                 int proj, int piter, hipStream_t st) {
+  hipLaunchKernelGGL((hyp3_level1<TC, TS>), grid_for(L.V, 6), dim3(256), 0, st, L, F, D,
+                     1.0 - a2, a2 / 2.0, proj, piter);
+  hipLaunchKernelGGL((hyp3_level2<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, F, D, out,
+                     1.0 - a1, a1 / 4.0, proj, piter);
+}
+
+template <class TC, class TS>
+void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
+                hipStream_t st) {
+  hipLaunchKernelGGL((ape_step<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, out, alpha,
+                     proj, piter);
+}
+
+void launch_ape4(const Lat& L, const double2* in, double2* out, double alpha, int proj, int piter,
+                 hipStream_t st) {
+  hipLaunchKernelGGL((ape_step4<double, double>), grid_for(L.V, 4), dim3(256), 0, st, L, in, out,
+                     alpha, proj, piter);
+}
+
+void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2* v, int mu, int nu,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(staple_sum_kernel, grid_for(L.V, 1), dim3(256), 0, st, L, A, B, v, mu, nu);
+}
+
+template void launch_hyp<double, double>(const Lat&, double2*, double2*, double2*, const double*,
+                                         int, int, int, int, int, hipStream_t);
+template void launch_hyp<float, double>(const Lat&, double2*, double2*, double2*, const double*,
+                                        int, int, int, int, int, hipStream_t);
+template void launch_hyp3<double, double>(const Lat&, const double2*, double2*, double2*, double,
+                                          double, int, int, hipStream_t);
+template void launch_hyp3<float, double>(const Lat&, const double2*, double2*, double2*, double,
+                                         double, int, int, hipStream_t);
+template void launch_ape<double, double>(const Lat&, const double2*, double2*, double, int, int,
+                                         hipStream_t);
+template void launch_ape<float, double>(const Lat&, const double2*, double2*, double, int, int,
+                                        hipStream_t);
+
+}  // namespace wl

@@ -1,0 +1,224 @@
+// su3.hpp -- 3x3 complex algebra and lattice indexing for the gfx950 kernels.
+//
+// Link fields live in HBM as structure-of-arrays "planes": a field with nl
+// links per site is an array of 2-vectors (re,im) indexed
+//     f[(l*9 + e)*V + s]
+// with l the link slot, e = 3*row + col the matrix element and s the
+// lexicographic site (x3 fastest).  A wavefront reading element e of 64
+// consecutive sites touches one contiguous 1 KiB (FP64) or 512 B (FP32) run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wl {
+
+template <class R> struct V2;
+template <> struct V2<double> { using type = double2; };
+template <> struct V2<float> { using type = float2; };
+template <class R> using v2 = typename V2<R>::type;
+
+// A 3x3 complex matrix held in registers, element e = 3*row + col.
+template <class R> struct M3 {
+  R re[9], im[9];
+};
+
+template <class R> __device__ __forceinline__ void set_zero(M3<R>& m) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) { m.re[e] = R(0); m.im[e] = R(0); }
+}
+
+template <class D, class S> __device__ __forceinline__ M3<D> convert(const M3<S>& a) {
+  M3<D> m;
+#pragma unroll
+  for (int e = 0; e < 9; e++) { m.re[e] = D(a.re[e]); m.im[e] = D(a.im[e]); }
+  return m;
+}
+
+// ---- HBM access ----------------------------------------------------------
+
+template <class R>
+__device__ __forceinline__ void load(M3<R>& m, const v2<R>* __restrict__ f, long V, int l, long s) {
+  const v2<R>* p = f + (long)l * 9 * V + s;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    v2<R> v = p[(long)e * V];
+    m.re[e] = v.x;
+    m.im[e] = v.y;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, long s) {
+  v2<R>* p = f + (long)l * 9 * V + s;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    v2<R> v;
+    v.x = m.re[e];
+    v.y = m.im[e];
+    p[(long)e * V] = v;
+  }
+}
+
+// ---- products (element order as the textbook sum over j = 0,1,2) ----------
+
+// w = a * b
+template <class R> __device__ __forceinline__ M3<R> mul(const M3<R>& a, const M3<R>& b) {
+  M3<R> w;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = a.re[3 * i] * b.re[k] - a.im[3 * i] * b.im[k];
+      R im = a.re[3 * i] * b.im[k] + a.im[3 * i] * b.re[k];
+      re += a.re[3 * i + 1] * b.re[3 + k] - a.im[3 * i + 1] * b.im[3 + k];
+      im += a.re[3 * i + 1] * b.im[3 + k] + a.im[3 * i + 1] * b.re[3 + k];
+      re += a.re[3 * i + 2] * b.re[6 + k] - a.im[3 * i + 2] * b.im[6 + k];
+      im += a.re[3 * i + 2] * b.im[6 + k] + a.im[3 * i + 2] * b.re[6 + k];
+      w.re[3 * i + k] = re;
+      w.im[3 * i + k] = im;
+    }
+  return w;
+}
+
+// w = a * b^dag
+template <class R> __device__ __forceinline__ M3<R> mul_dag(const M3<R>& a, const M3<R>& b) {
+  M3<R> w;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = a.re[3 * i] * b.re[3 * k] + a.im[3 * i] * b.im[3 * k];
+      R im = a.im[3 * i] * b.re[3 * k] - a.re[3 * i] * b.im[3 * k];
+      re += a.re[3 * i + 1] * b.re[3 * k + 1] + a.im[3 * i + 1] * b.im[3 * k + 1];
+      im += a.im[3 * i + 1] * b.re[3 * k + 1] - a.re[3 * i + 1] * b.im[3 * k + 1];
+      re += a.re[3 * i + 2] * b.re[3 * k + 2] + a.im[3 * i + 2] * b.im[3 * k + 2];
+      im += a.im[3 * i + 2] * b.re[3 * k + 2] - a.re[3 * i + 2] * b.im[3 * k + 2];
+      w.re[3 * i + k] = re;
+      w.im[3 * i + k] = im;
+    }
+  return w;
+}
+
+// w = a^dag * b
+template <class R> __device__ __forceinline__ M3<R> dag_mul(const M3<R>& a, const M3<R>& b) {
+  M3<R> w;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = a.re[i] * b.re[k] + a.im[i] * b.im[k];
+      R im = a.re[i] * b.im[k] - a.im[i] * b.re[k];
+      re += a.re[3 + i] * b.re[3 + k] + a.im[3 + i] * b.im[3 + k];
+      im += a.re[3 + i] * b.im[3 + k] - a.im[3 + i] * b.re[3 + k];
+      re += a.re[6 + i] * b.re[6 + k] + a.im[6 + i] * b.im[6 + k];
+      im += a.re[6 + i] * b.im[6 + k] - a.im[6 + i] * b.re[6 + k];
+      w.re[3 * i + k] = re;
+      w.im[3 * i + k] = im;
+    }
+  return w;
+}
+
+template <class R> __device__ __forceinline__ void acc(M3<R>& v, const M3<R>& a) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) { v.re[e] += a.re[e]; v.im[e] += a.im[e]; }
+}
+
+// Re tr(a^dag b), row by row
+template <class R> __device__ __forceinline__ R retr_dag_mul(const M3<R>& a, const M3<R>& b) {
+  R r[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    R t = a.re[3 * i] * b.re[3 * i] + a.im[3 * i] * b.im[3 * i];
+    t += a.re[3 * i + 1] * b.re[3 * i + 1] + a.im[3 * i + 1] * b.im[3 * i + 1];
+    t += a.re[3 * i + 2] * b.re[3 * i + 2] + a.im[3 * i + 2] * b.im[3 * i + 2];
+    r[i] = t;
+  }
+  return r[0] + r[1] + r[2];
+}
+
+// ---- approximate SU(3) projection of hep-lat/0506008, always FP64 ---------
+// Scale to unit Frobenius norm/3, then p times
+//     X = u (3 - u^dag u)/2 ;  u = (1 - i Im det X / 3) X .
+// Matrices with sum |u_ab|^2 < sqrt(18)*DBL_EPSILON are returned untouched
+// (this keeps the zero links of open boundaries zero).
+__device__ __forceinline__ double det_im(const M3<double>& u) {
+  const double* r = u.re;
+  const double* i = u.im;
+  double d1r = (r[4] * r[8] - i[4] * i[8]) - (r[5] * r[7] - i[5] * i[7]);
+  double d1i = (r[4] * i[8] + i[4] * r[8]) - (r[5] * i[7] + i[5] * r[7]);
+  double d2r = (r[3] * r[8] - i[3] * i[8]) - (r[5] * r[6] - i[5] * i[6]);
+  double d2i = (r[3] * i[8] + i[3] * r[8]) - (r[5] * i[6] + i[5] * r[6]);
+  double d3r = (r[3] * r[7] - i[3] * i[7]) - (r[4] * r[6] - i[4] * i[6]);
+  double d3i = (r[3] * i[7] + i[3] * r[7]) - (r[4] * i[6] + i[4] * r[6]);
+  return (r[0] * d1i + i[0] * d1r) - (r[1] * d2i + i[1] * d2r) + (r[2] * d3i + i[2] * d3r);
+}
+
+__device__ __forceinline__ void project(M3<double>& u, int iter) {
+  double d = 0.0;
+#pragma unroll
+  for (int e = 0; e < 9; e++) d += u.re[e] * u.re[e] + u.im[e] * u.im[e];
+  if (d < 4.242640687119285 * 2.220446049250313e-16) return;
+  d = 1.0 / sqrt(d / 3.0);
+#pragma unroll
+  for (int e = 0; e < 9; e++) { u.re[e] *= d; u.im[e] *= d; }
+  for (int it = 0; it < iter; it++) {
+    // h = u^dag u is Hermitian: build the 6 independent entries
+    M3<double> x;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = a; b < 3; b++) {
+        double re = u.re[a] * u.re[b] + u.im[a] * u.im[b];
+        double im = u.re[a] * u.im[b] - u.im[a] * u.re[b];
+        re += u.re[3 + a] * u.re[3 + b] + u.im[3 + a] * u.im[3 + b];
+        im += u.re[3 + a] * u.im[3 + b] - u.im[3 + a] * u.re[3 + b];
+        re += u.re[6 + a] * u.re[6 + b] + u.im[6 + a] * u.im[6 + b];
+        im += u.re[6 + a] * u.im[6 + b] - u.im[6 + a] * u.re[6 + b];
+        if (a == b) {
+          x.re[3 * a + a] = 1.5 - 0.5 * re;
+          x.im[3 * a + a] = 0.0;
+        } else {
+          x.re[3 * a + b] = -0.5 * re;
+          x.im[3 * a + b] = -0.5 * im;
+          x.re[3 * b + a] = -0.5 * re;
+          x.im[3 * b + a] = 0.5 * im;
+        }
+      }
+    x = mul(u, x);
+    double zi = det_im(x) * (-1.0 / 3.0);
+#pragma unroll
+    for (int e = 0; e < 9; e++) {
+      u.re[e] = x.re[e] - zi * x.im[e];
+      u.im[e] = x.im[e] + zi * x.re[e];
+    }
+  }
+}
+
+// ---- lattice geometry ------------------------------------------------------
+// Lexicographic site s = x3 + N3*(x2 + N2*(x1 + N1*x0)); every direction wraps
+// periodically in index space (open boundaries are zero time links).
+struct Lat {
+  int n[4];
+  long st[4];  // strides: N1N2N3, N2N3, N3, 1
+  long V;
+};
+
+__device__ __forceinline__ void coords(const Lat& L, long s, int x[4]) {
+  long r = s;
+  x[3] = (int)(r % L.n[3]); r /= L.n[3];
+  x[2] = (int)(r % L.n[2]); r /= L.n[2];
+  x[1] = (int)(r % L.n[1]); r /= L.n[1];
+  x[0] = (int)r;
+}
+
+// site reached from s (coordinates x) by d >= 0 steps in direction mu
+__device__ __forceinline__ long fwd(const Lat& L, long s, const int x[4], int mu, int d) {
+  int y = x[mu] + d;
+  return (y >= L.n[mu]) ? s + (long)(d - L.n[mu]) * L.st[mu] : s + (long)d * L.st[mu];
+}
+
+__device__ __forceinline__ long bwd(const Lat& L, long s, const int x[4], int mu) {
+  return (x[mu] == 0) ? s + (long)(L.n[mu] - 1) * L.st[mu] : s - L.st[mu];
+}
+
+}  // namespace wl
