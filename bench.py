@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the Wilson-loop measurement path on one MI355X.
+
+Workload (one "step" = one configuration measured end to end):
+    48^3 x 96 lattice, open boundaries in time, seeded synthetic links
+    U = exp(i eps H) (eps = 0.25) under a Haar gauge transformation;
+    HYP of the static action (alpha = 1.0, 1.0, 0.5), APE operator levels
+    {0,16,48,96} (nss = 0 16 32 48, alpha = 0.5, proj_iter = 4),
+    W_ij(R,T;x0) for R,T = 1..16.
+The field is generated on the device before timing, so inputs are resident in
+HBM when the clock starts; each step runs HYP -> levels -> transporters ->
+contraction and copies W (3.1 MB) to the host.
+
+Paths: "mixed" (FP32 staples and transporter products, FP64 projection, FP64
+time-transporter chain beyond T = 8, FP64 sums) is the headline value; the
+plain FP64 path is timed beside it, and the mixed W is compared with it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "c3": dict(dims=(96, 48, 48, 48), bc=0, eps=0.25, seed=3, nss=[0, 16, 32, 48], R=16, T=16,
+               name="48^3x96 open bc, eps=0.25, HYP(1,1,0.5) + APE levels {0,16,48,96}, R,T<=16"),
+    "c2": dict(dims=(48, 24, 24, 24), bc=3, eps=0.3, seed=2, nss=[0, 8, 16, 24], R=12, T=12,
+               name="24^3x48 periodic, eps=0.3, HYP(1,1,0.5) + APE levels {0,8,24,48}, R,T<=12"),
+    "c1": dict(dims=(8, 8, 8, 8), bc=3, eps=0.4, seed=1, nss=[0, 4, 4], R=4, T=4,
+               name="8^4 periodic, eps=0.4, HYP(1,1,0.5) + APE levels {0,4,8}, R,T<=4"),
+}
+ALPHA_ACTION = (1.0, 1.0, 0.5)
+ALPHA_APE = 0.5
+PROJ_ITER = 4
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_err(got, ref):
+    zero = ref == 0.0
+    if np.any(got[zero] != 0.0):
+        return float("inf")
+    nz = ~zero
+    return float(np.max(np.abs(got[nz] - ref[nz]) / np.abs(ref[nz]))) if nz.any() else 0.0
+
+
+def time_path(lat, parms, precision, steps, warmup):
+    for _ in range(warmup):
+        lat.measure(parms, precision)
+    lat.synchronize()
+    lat.timers_reset()
+    lat_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        w = lat.measure(parms, precision)
+        lat_ms.append((time.perf_counter() - ts) * 1e3)
+    lat.synchronize()
+    total = time.perf_counter() - t0
+    return w, total, lat_ms, lat.timers(), lat.contract_stats()
+
+
+def cpu_baseline(wl_cfg, parms_o):
+    """Oracle (the reference loop structure, FP64) timed on this host's cores on a
+    48^3 x 2 slab of the same workload, extrapolated per site and per stage count."""
+    import oracle as O
+    dims = wl_cfg["dims"]
+    sdims = (2,) + tuple(dims[1:])
+    u = O.generate(sdims, 3, wl_cfg["seed"], wl_cfg["eps"])
+    cores = min(16, len(os.sched_getaffinity(0)))
+    t = O.time_stages(sdims, 3, u, ALPHA_ACTION, parms_o, cores)
+    f = float(np.prod(dims)) / float(np.prod(sdims))
+    steps = sum(wl_cfg["nss"])
+    R, T = wl_cfg["R"], wl_cfg["T"]
+    t_cfg = f * (t[0] + steps * t[1] + R * (T - 1) * t[2] + (R - 1) * t[3] + R * T * t[4])
+    sample = (f"oracle stages on a {sdims[1]}^3x{sdims[0]} slab ({t.sum():.1f} s: 1 HYP, 1 APE step, "
+              f"1 L_T update, 1 S_R update, 1 (R,T) contraction), scaled by {f:.0f}x volume and "
+              f"stage counts ({steps} APE steps, {R}x{T - 1} L_T, {R - 1} S_R, {R * T} contractions)")
+    return {"value": 1.0 / t_cfg, "unit": "configs/s", "cores": cores, "kind": "port",
+            "sample": sample, "seconds_per_config": t_cfg}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
+              "part of this build: independent configurations shard one per GPU with no "
+              "communication.", file=sys.stderr)
+        sys.exit(2)
+
+    import wloop_amd as W
+    cfg = WORKLOADS[args.workload]
+    dims = cfg["dims"]
+    parms = W.make_parms(cfg["nss"], cfg["T"], cfg["R"], alpha_ape=ALPHA_APE, proj_iter=PROJ_ITER,
+                         alpha_action=ALPHA_ACTION)
+    log(f"[bench] workload {args.workload}: {cfg['name']}")
+    lat = W.Lattice(dims, cfg["bc"])
+    lat.generate(cfg["seed"], cfg["eps"])
+    lat.synchronize()
+    log("[bench] field generated on device")
+
+    w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
+    log(f"[bench] fp64: {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
+    wmx, tmx, latmx, timmx, csmx = time_path(lat, parms, W.MIXED, args.steps, args.warmup)
+    log(f"[bench] mixed: {tmx / args.steps * 1e3:.1f} ms/config  stages {timmx}")
+    err = rel_err(wmx, w64)
+    log(f"[bench] mixed vs fp64 max rel err {err:.3e}")
+
+    V = int(np.prod(dims))
+    links_per_site = 1 + 3 * parms.msl            # L_T + S_R of every level and direction
+
+    def roof(cs, link_bytes):
+        n, ms = cs
+        avg_s = ms / n / 1e3
+        algo = V * links_per_site * link_bytes
+        ach = algo / avg_s / 1e9
+        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "contract",
+                "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": algo}
+
+    roof_mx = roof(csmx, 72)
+    roof_64 = roof(cs64, 144)
+    prof = os.path.join(ROOT, "profiles", "contract_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        roof_mx["traffic"] = tr.get("mixed_bytes_per_launch")
+        roof_64["traffic"] = tr.get("fp64_bytes_per_launch")
+
+    out = {
+        "metric": "configs/s at 48^3x96 (mixed vs FP64 GPU, rel err<1e-6); p50/p99 latency",
+        "value": args.steps / tmx,
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": tmx / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded Philox links exp(i eps H) with Haar gauge transform, generated on device)",
+        "config": {"workload": cfg["name"], "lattice": list(dims), "bc": "open" if cfg["bc"] == 0 else "periodic",
+                   "seed": cfg["seed"], "nss": cfg["nss"], "mwlt": cfg["T"], "mwlr": cfg["R"],
+                   "proj_iter": PROJ_ITER, "alpha_action": list(ALPHA_ACTION), "alpha_ape": ALPHA_APE,
+                   "parallelism": "1 GPU"},
+        "latency_ms": {"p50": float(np.percentile(latmx, 50)), "p99": float(np.percentile(latmx, 99))},
+        "fp64": {"value": args.steps / t64, "ms_per_step": t64 / args.steps * 1e3,
+                 "latency_ms": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
+                 "stage_ms_per_config": {k: v / args.steps for k, v in tim64.items()},
+                 "roofline": roof_64},
+        "mixed_over_fp64": t64 / tmx,
+        "mixed_max_rel_err_vs_fp64": err,
+        "stage_ms_per_config": {k: v / args.steps for k, v in timmx.items()},
+        "roofline": roof_mx,
+    }
+    if not args.no_cpu_baseline:
+        po = None
+        import oracle as O
+        po = O.make_parms(cfg["nss"], cfg["T"], cfg["R"], alpha_ape=ALPHA_APE, proj_iter=PROJ_ITER)
+        out["cpu_baseline"] = cpu_baseline(cfg, po)
+        log(f"[bench] cpu baseline {out['cpu_baseline']}")
+    else:
+        out["cpu_baseline"] = None
+    lat.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

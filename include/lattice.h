@@ -1,0 +1,53 @@
+/*
+ * lattice.h -- the openQCD-1.4 pieces the measurement path calls, supplied by
+ * libwloop_amd.so for the standalone build (openQCD is not part of it).
+ * Names and meaning follow openQCD: geometry() builds ipt/iup/idn, udfld()
+ * returns the double-precision gauge field (4*VOLUME su3_dble; link U(x,mu)
+ * for odd x at [8*(ix-VOLUME/2)+2*mu], for even x at the odd neighbour
+ * x+mu, slot +2*mu+1 -- modules/lnk/lnkcom.c:840-859), set_bc() zeroes the
+ * time links at x0 = N0-1 for open boundary conditions.
+ *
+ * The gauge field also lives on the GPU; the library keeps the two copies
+ * coherent: udfld() hands out the host copy (downloading it first if the
+ * device holds newer links) and the next GPU operation uploads it again.
+ */
+#ifndef LATTICE_H
+#define LATTICE_H
+
+#include <stdio.h>
+#include "su3.h"
+#include "global.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* replaces openQCD's compile-time L0..L3 (global.h); extents must be even */
+void set_lattice_sizes(int n0, int n1, int n2, int n3);
+void geometry(void);
+
+/* boundary conditions: 0 open, 3 periodic (the two the reference supports,
+ * README:22); the other openQCD parameters are not used by the path */
+void set_bc_type(int type);
+int bc_type(void);
+void set_bc(void);
+
+su3_dble *udfld(void);
+void copy_bnd_ud(void);
+
+#define UPDATED_UD 1
+void set_flags(int event);
+
+/* openQCD-style error reporting: print to stdout and abort */
+void error(int test, int no, const char *name, const char *format, ...);
+void error_root(int test, int no, const char *name, const char *format, ...);
+void error_chk(void);
+void message(const char *format, ...);
+
+/* GPU device used by the module entry points (default 0) */
+void set_wloop_device(int device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -58,14 +58,21 @@ int wlg_upload(wlg_ctx *ctx, const double *u);
 int wlg_download(wlg_ctx *ctx, double *u);
 int wlg_generate(wlg_ctx *ctx, uint64_t seed, double eps);
 
-/* HYP smearing of the resident gauge field, in place (hyp_links.c:398-472) */
-int wlg_hyp(wlg_ctx *ctx, const double alpha[3], int proj_iter, int time, int space,
+/* HYP smearing of the resident gauge field, in place (hyp_links.c:398-472);
+ * proj != 0 applies approx_project_to_su3_dble(piter) after each level */
+int wlg_hyp(wlg_ctx *ctx, const double alpha[3], int proj, int piter, int time, int space,
             int precision);
 /* one APE step of the resident field in place: spatial (four_d = 0,
  * ape_spatial_links) or all four directions (ape_links), ape_links.c:217-284 */
-int wlg_ape_step(wlg_ctx *ctx, double alpha, int proj_iter, int four_d, int precision);
+int wlg_ape_step(wlg_ctx *ctx, double alpha, int proj, int piter, int four_d, int precision);
 /* one 3-d HYP step of the resident field's spatial links (hyp_spatial_links.c) */
-int wlg_hyp_spatial_step(wlg_ctx *ctx, double a1, double a2, int proj_iter, int precision);
+int wlg_hyp_spatial_step(wlg_ctx *ctx, double a1, double a2, int proj, int piter,
+                         int precision);
+/* U_0(x0 = N0-1) = 0 on the resident field if the context has open bc (set_bc) */
+int wlg_set_bc(wlg_ctx *ctx);
+/* sum over x and planes of Re tr U_mu(x)U_nu(x+mu)(U_nu(x)U_mu(x+nu))^dag, time
+ * plaquettes at x0 = N0-1 left out for open bc (lnkplaq.c:68-128) */
+int wlg_plaquette(wlg_ctx *ctx, double *sum);
 /* v_mu += S_{mu nu}[a,b] on host AoS fields [V][4][18] (staple_sum.c:53-87) */
 int wlg_staple_sum(wlg_ctx *ctx, const double *a, const double *b, int mu, int nu, double *v);
 
@@ -75,13 +82,18 @@ int wlg_staple_sum(wlg_ctx *ctx, const double *a, const double *b, int mu, int n
 int wlg_wloop(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
 /* ms10 set_data (ms10.c:633-646): HYP with alpha_action, then wlg_wloop */
 int wlg_measure(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
-/* spatial links of operator level sl of the last wlg_wloop, host AoS [V][3][18] */
+/* operator levels only (smear.c:99-121) of the resident field */
+int wlg_levels(wlg_ctx *ctx, const wlg_parms *p, int precision);
+/* spatial links of operator level sl of the last wlg_levels/wlg_wloop/wlg_measure,
+ * host AoS [V][3][18] */
 int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
 
 /* device bytes one measurement needs (admission by HBM footprint) */
 size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 
 int wlg_synchronize(wlg_ctx *ctx);
+/* release the scratch memory of the context (the resident field stays) */
+int wlg_trim(wlg_ctx *ctx);
 void wlg_timers_reset(wlg_ctx *ctx);
 double wlg_timer_ms(wlg_ctx *ctx, int stage);
 /* number of contraction launches and their summed duration (ms), measured with

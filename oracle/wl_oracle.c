@@ -37,6 +37,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -439,6 +440,95 @@ void wo_levels(const int *dims, const double *u, const wo_parms *p, double *lev)
 
 static inline const su3 *C3(const double *l, long s, int k) { return (const su3 *)(l + 18 * (3 * s + k)); }
 
+/* time transporter update L_T(x) = U_0(x) L_{T-1}(x+0) (wloop.c:153-204) */
+static void lt_update(const geom *g, const double *u, const double *lt, double *out)
+{
+#pragma omp parallel for schedule(static)
+   for (long x = 0; x < g->vol; x++)
+      su3_mul(C4(u, x, 0), (const su3 *)(lt + 18 * g->up[4 * x + 0]), (su3 *)(out + 18 * x));
+}
+
+/* spatial transporter update S_R(x,k) = F_k(x) S_{R-1}(x+k,k) (wloop.c:207-258) */
+static void sr_update(const geom *g, const double *f, const double *prev, double *out)
+{
+#pragma omp parallel for schedule(static)
+   for (long x = 0; x < g->vol; x++)
+      for (int k = 0; k < 3; k++)
+         su3_mul(C3(f, x, k), C3(prev, g->up[4 * x + k + 1], k), (su3 *)(out + 18 * (3 * x + k)));
+}
+
+/* One (R,T) block of the contraction into wl (wloop.c:436-508, :558-600).
+ * mode 0 repeats the reference loop structure, mode 1 shares the products. */
+static void contract_rt(const geom *g, const wo_parms *p, int r, int t, const double *lt,
+                        const double *S, int mode, double *acc, double *wl)
+{
+   const int msl = p->msl, N0 = g->n[0], R = r + 1, T = t + 1;
+   const long slice = g->vol / N0;
+   const double norm = 9.0 * g->n[1] * g->n[2] * g->n[3];    /* wloop.c:535 */
+   const size_t lstride = (size_t)18 * 3 * g->vol;
+   if (mode == 0) {
+      double *res = acc;
+      for (int sl1 = 0; sl1 < msl; sl1++)
+         for (int sl2 = sl1; sl2 < msl; sl2++) {
+            const double *s1 = S + sl1 * lstride, *s2 = S + sl2 * lstride;
+#pragma omp parallel for schedule(static)
+            for (int x0 = 0; x0 < N0; x0++) {
+               double r0 = 0.0, r1 = 0.0;
+               long xo = (long)((x0 + T) % N0) * slice;
+               for (long xs = 0; xs < slice; xs++) {
+                  long x = (long)x0 * slice + xs;
+                  const su3 *ub = (const su3 *)(lt + 18 * x);
+                  for (int k = 0; k < 3; k++) {
+                     long xr = x;
+                     for (int q = 0; q < R; q++) xr = g->up[4 * xr + k + 1];
+                     const su3 *ubr = (const su3 *)(lt + 18 * xr);
+                     su3 m0, m1;
+                     su3_mul(ub, C3(s2, xo + xs, k), &m0);
+                     su3_mul(C3(s1, x, k), ubr, &m1);
+                     r0 += su3_retr_dag_mul(&m0, &m1);
+                     if (sl1 != sl2) {
+                        su3_mul(ub, C3(s1, xo + xs, k), &m0);
+                        su3_mul(C3(s2, x, k), ubr, &m1);
+                        r1 += su3_retr_dag_mul(&m0, &m1);
+                     }
+                  }
+               }
+               res[x0] = r0;
+               res[x0 + N0] = r1;
+            }
+            for (int x0 = 0; x0 < N0; x0++) {                  /* wloop.c:584-597 */
+               wl[wo_wl_idx(p, sl1, sl2, t, r, x0)] = res[x0] / norm;
+               if (sl1 != sl2) wl[wo_wl_idx(p, sl2, sl1, t, r, x0)] = res[x0 + N0] / norm;
+            }
+         }
+   } else {
+#pragma omp parallel for schedule(static)
+      for (int x0 = 0; x0 < N0; x0++) {
+         double *a = acc + (size_t)x0 * msl * msl;
+         for (int q = 0; q < msl * msl; q++) a[q] = 0.0;
+         long xo = (long)((x0 + T) % N0) * slice;
+         su3 A[5], B[5];
+         for (long xs = 0; xs < slice; xs++) {
+            long x = (long)x0 * slice + xs;
+            const su3 *ub = (const su3 *)(lt + 18 * x);
+            for (int k = 0; k < 3; k++) {
+               long xr = x;
+               for (int q = 0; q < R; q++) xr = g->up[4 * xr + k + 1];
+               const su3 *ubr = (const su3 *)(lt + 18 * xr);
+               for (int j = 0; j < msl; j++) su3_mul(ub, C3(S + j * lstride, xo + xs, k), &A[j]);
+               for (int i = 0; i < msl; i++) su3_mul(C3(S + i * lstride, x, k), ubr, &B[i]);
+               for (int i = 0; i < msl; i++)
+                  for (int j = 0; j < msl; j++) a[i * msl + j] += su3_retr_dag_mul(&A[j], &B[i]);
+            }
+         }
+      }
+      for (int x0 = 0; x0 < N0; x0++)
+         for (int i = 0; i < msl; i++)
+            for (int j = 0; j < msl; j++)
+               wl[wo_wl_idx(p, i, j, t, r, x0)] = acc[(size_t)x0 * msl * msl + i * msl + j] / norm;
+   }
+}
+
 void wo_wloop(const int *dims, const double *u, const wo_parms *p, int mode, int nthreads,
               double *wl)
 {
@@ -452,8 +542,6 @@ void wo_wloop(const int *dims, const double *u, const wo_parms *p, int mode, int
    geom_init(&g, dims);
    const int msl = p->msl;
    const int N0 = g.n[0];
-   const long slice = g.vol / N0;
-   const double norm = 9.0 * g.n[1] * g.n[2] * g.n[3];  /* wloop.c:535 */
    size_t lsz = sizeof(double) * 18 * 3 * g.vol;
 
    double *lev = malloc(lsz * msl);                     /* F^(sl), the pus[sl] table */
@@ -462,8 +550,7 @@ void wo_wloop(const int *dims, const double *u, const wo_parms *p, int mode, int
    double *stmp = malloc(lsz);
    double *lt = malloc(sizeof(su3) * g.vol);            /* ud_long time links */
    double *ltmp = malloc(sizeof(su3) * g.vol);
-   double *res = malloc(sizeof(double) * 2 * N0);
-   double *acc = malloc(sizeof(double) * N0 * msl * msl);
+   double *acc = malloc(sizeof(double) * (N0 * msl * msl + 2 * N0));
 
    for (int r = 0; r < p->mwlr; r++) {
       const int R = r + 1;
@@ -471,94 +558,89 @@ void wo_wloop(const int *dims, const double *u, const wo_parms *p, int mode, int
          memcpy(lt + 18 * x, C4(u, x, 0), sizeof(su3));
       for (int t = 0; t < p->mwlt; t++) {
          const int T = t + 1;
-         if (T > 1) {                                     /* compute_u0_long, wloop.c:153-204 */
-#pragma omp parallel for schedule(static)
-            for (long x = 0; x < g.vol; x++)
-               su3_mul(C4(u, x, 0), (const su3 *)(lt + 18 * g.up[4 * x + 0]),
-                       (su3 *)(ltmp + 18 * x));
+         if (T > 1) {                                     /* recomputed per R, wloop.c:554-556 */
+            lt_update(&g, u, lt, ltmp);
             double *sw = lt; lt = ltmp; ltmp = sw;
          }
-         if (R > 1 && T == 1) {                           /* compute_uk_long, wloop.c:207-258 */
+         if (R > 1 && T == 1) {                           /* wloop.c:562-563 */
             for (int sl = 0; sl < msl; sl++) {
                const double *f = lev + (size_t)sl * 18 * 3 * g.vol;
                const double *prev = (R == 2) ? f : slong + (size_t)sl * 18 * 3 * g.vol;
-#pragma omp parallel for schedule(static)
-               for (long x = 0; x < g.vol; x++)
-                  for (int k = 0; k < 3; k++)
-                     su3_mul(C3(f, x, k), C3(prev, g.up[4 * x + k + 1], k),
-                             (su3 *)(stmp + 18 * (3 * x + k)));
+               sr_update(&g, f, prev, stmp);
                memcpy(slong + (size_t)sl * 18 * 3 * g.vol, stmp, lsz);
             }
          }
-         const double *S = (R > 1) ? slong : lev;
-         const size_t lstride = (size_t)18 * 3 * g.vol;
-
-         if (mode == 0) {
-            /* reference structure: wloop_fixed_t_r_sl per level pair
-             * (wloop.c:436-508, loop nest wloop.c:558-600) */
-            for (int sl1 = 0; sl1 < msl; sl1++)
-               for (int sl2 = sl1; sl2 < msl; sl2++) {
-                  const double *s1 = S + sl1 * lstride, *s2 = S + sl2 * lstride;
-#pragma omp parallel for schedule(static)
-                  for (int x0 = 0; x0 < N0; x0++) {
-                     double r0 = 0.0, r1 = 0.0;
-                     long xo = (long)((x0 + T) % N0) * slice;
-                     for (long xs = 0; xs < slice; xs++) {
-                        long x = (long)x0 * slice + xs;
-                        const su3 *ub = (const su3 *)(lt + 18 * x);
-                        for (int k = 0; k < 3; k++) {
-                           long xr = x;
-                           for (int q = 0; q < R; q++) xr = g.up[4 * xr + k + 1];
-                           const su3 *ubr = (const su3 *)(lt + 18 * xr);
-                           su3 m0, m1;
-                           su3_mul(ub, C3(s2, xo + xs, k), &m0);
-                           su3_mul(C3(s1, x, k), ubr, &m1);
-                           r0 += su3_retr_dag_mul(&m0, &m1);
-                           if (sl1 != sl2) {
-                              su3_mul(ub, C3(s1, xo + xs, k), &m0);
-                              su3_mul(C3(s2, x, k), ubr, &m1);
-                              r1 += su3_retr_dag_mul(&m0, &m1);
-                           }
-                        }
-                     }
-                     res[x0] = r0;
-                     res[x0 + N0] = r1;
-                  }
-                  for (int x0 = 0; x0 < N0; x0++) {         /* wloop.c:584-597 */
-                     wl[wo_wl_idx(p, sl1, sl2, t, r, x0)] = res[x0] / norm;
-                     if (sl1 != sl2) wl[wo_wl_idx(p, sl2, sl1, t, r, x0)] = res[x0 + N0] / norm;
-                  }
-               }
-         } else {
-            /* shared products: A_j = L_T(x) S_j(x+T0), B_i = S_i(x) L_T(x+Rk) */
-#pragma omp parallel for schedule(static)
-            for (int x0 = 0; x0 < N0; x0++) {
-               double *a = acc + (size_t)x0 * msl * msl;
-               for (int q = 0; q < msl * msl; q++) a[q] = 0.0;
-               long xo = (long)((x0 + T) % N0) * slice;
-               su3 A[5], B[5];
-               for (long xs = 0; xs < slice; xs++) {
-                  long x = (long)x0 * slice + xs;
-                  const su3 *ub = (const su3 *)(lt + 18 * x);
-                  for (int k = 0; k < 3; k++) {
-                     long xr = x;
-                     for (int q = 0; q < R; q++) xr = g.up[4 * xr + k + 1];
-                     const su3 *ubr = (const su3 *)(lt + 18 * xr);
-                     for (int j = 0; j < msl; j++) su3_mul(ub, C3(S + j * lstride, xo + xs, k), &A[j]);
-                     for (int i = 0; i < msl; i++) su3_mul(C3(S + i * lstride, x, k), ubr, &B[i]);
-                     for (int i = 0; i < msl; i++)
-                        for (int j = 0; j < msl; j++) a[i * msl + j] += su3_retr_dag_mul(&A[j], &B[i]);
-                  }
-               }
-            }
-            for (int x0 = 0; x0 < N0; x0++)
-               for (int i = 0; i < msl; i++)
-                  for (int j = 0; j < msl; j++)
-                     wl[wo_wl_idx(p, i, j, t, r, x0)] = acc[(size_t)x0 * msl * msl + i * msl + j] / norm;
-         }
+         contract_rt(&g, p, r, t, lt, (R > 1) ? slong : lev, mode, acc, wl);
       }
    }
-   free(acc); free(res); free(ltmp); free(lt); free(stmp); free(slong); free(lev);
+   free(acc); free(ltmp); free(lt); free(stmp); free(slong); free(lev);
+   geom_free(&g);
+#ifdef _OPENMP
+   omp_set_num_threads(saved);
+#endif
+}
+
+static double now_s(void)
+{
+#ifdef _OPENMP
+   return omp_get_wtime();
+#else
+   return (double)clock() / CLOCKS_PER_SEC;
+#endif
+}
+
+/* Wall seconds of one instance of each stage of the reference loop structure
+ * on lattice dims (CPU baseline sampling):
+ *   t[0] hyp_links on u; t[1] one APE step (or 3-d HYP step);
+ *   t[2] one time-transporter update; t[3] one spatial-transporter update of
+ *   all levels; t[4] one (R,T) contraction over all level pairs (mode 0),
+ *   mean of (R,T) = (1,1) and (mwlr,mwlt). */
+void wo_time_stages(const int *dims, int bc, const double *u, const double *aa,
+                    const wo_parms *p, int nthreads, double *t)
+{
+#ifdef _OPENMP
+   int saved = omp_get_max_threads();
+   if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+   geom g;
+   geom_init(&g, dims);
+   const int msl = p->msl;
+   size_t fsz = sizeof(double) * 18 * 4 * g.vol, lsz = sizeof(double) * 18 * 3 * g.vol;
+   double *w = malloc(fsz), *work = malloc(2 * fsz);
+   memcpy(w, u, fsz);
+   double t0 = now_s();
+   wo_hyp(dims, bc, w, aa[0], aa[1], aa[2], p->proj, p->proj_iter, 1, 1);
+   t[0] = now_s() - t0;
+   double *f = malloc(fsz);
+   memcpy(f, w, fsz);
+   t0 = now_s();
+   if (p->op_smear == 1)
+      wo_hyp_spatial(dims, f, p->alpha_hyp[0], p->alpha_hyp[1], p->proj, p->proj_iter);
+   else
+      ape_step(&g, f, work, p->alpha_ape, p->proj, p->proj_iter, 0);
+   t[1] = now_s() - t0;
+   double *lev = malloc(lsz * msl), *lt = malloc(sizeof(su3) * g.vol), *lt2 = malloc(sizeof(su3) * g.vol);
+   for (int sl = 0; sl < msl; sl++)
+      for (long x = 0; x < g.vol; x++)
+         for (int k = 0; k < 3; k++)
+            memcpy(lev + (size_t)sl * 18 * 3 * g.vol + 18 * (3 * x + k), L4(f, x, k + 1), sizeof(su3));
+   for (long x = 0; x < g.vol; x++) memcpy(lt + 18 * x, C4(w, x, 0), sizeof(su3));
+   t0 = now_s();
+   lt_update(&g, w, lt, lt2);
+   t[2] = now_s() - t0;
+   double *sl2 = malloc(lsz * msl);
+   t0 = now_s();
+   for (int sl = 0; sl < msl; sl++)
+      sr_update(&g, lev + (size_t)sl * 18 * 3 * g.vol, lev + (size_t)sl * 18 * 3 * g.vol,
+                sl2 + (size_t)sl * 18 * 3 * g.vol);
+   t[3] = now_s() - t0;
+   double *acc = malloc(sizeof(double) * (g.n[0] * msl * msl + 2 * g.n[0]));
+   double *wl = malloc(sizeof(double) * msl * msl * p->mwlt * p->mwlr * g.n[0]);
+   t0 = now_s();
+   contract_rt(&g, p, 0, 0, lt2, sl2, 0, acc, wl);
+   contract_rt(&g, p, p->mwlr - 1, p->mwlt - 1, lt2, sl2, 0, acc, wl);
+   t[4] = 0.5 * (now_s() - t0);
+   free(wl); free(acc); free(sl2); free(lt2); free(lt); free(lev); free(f); free(work); free(w);
    geom_free(&g);
 #ifdef _OPENMP
    omp_set_num_threads(saved);

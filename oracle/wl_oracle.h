@@ -69,6 +69,13 @@ void wo_ms10(const int *dims, int bc, double *u, const double *alpha_action,
 /* Spatial operator level sl (3V links, [V][3][18]) as built by smear(). */
 void wo_levels(const int *dims, const double *u, const wo_parms *p, double *lev);
 
+/* Wall seconds of one instance of each stage, reference loop structure:
+ * t[0] HYP, t[1] one APE (or 3-d HYP) step, t[2] one time-transporter update,
+ * t[3] one spatial-transporter update of all levels, t[4] one (R,T)
+ * contraction over all level pairs (mean of (1,1) and (mwlr,mwlt)). */
+void wo_time_stages(const int *dims, int bc, const double *u, const double *alpha_action,
+                    const wo_parms *p, int nthreads, double *t);
+
 /* ---- test helpers (openQCD random_g/transform_ud/shift_ud analogues) ---- */
 void wo_gauge_transform(const int *dims, double *u, uint64_t seed);
 void wo_shift(const int *dims, double *u, const int *s);

@@ -70,6 +70,8 @@ def lib():
         _lib.wo_random_su3.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, dp]
         _lib.wo_philox4x32.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
         _lib.wo_set_threads.argtypes = [ctypes.c_int]
+        _lib.wo_time_stages.argtypes = [ip, ctypes.c_int, dp, dp, ctypes.POINTER(Parms),
+                                        ctypes.c_int, dp]
     return _lib
 
 
@@ -204,3 +206,13 @@ def from_complex(c):
     out[..., 0::2] = f.real
     out[..., 1::2] = f.imag
     return out
+
+
+def time_stages(dims, bc, u, alpha_action, parms, nthreads=0):
+    """Seconds of one HYP, one APE step, one L_T update, one S_R update (all
+    levels) and one (R,T) contraction of the reference loop structure."""
+    t = np.zeros(5)
+    aa = np.asarray(alpha_action, dtype=np.float64)
+    lib().wo_time_stages(_dims(dims), bc, _dp(np.ascontiguousarray(u)), _dp(aa),
+                         ctypes.byref(parms), nthreads, _dp(t))
+    return t

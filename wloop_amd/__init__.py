@@ -68,9 +68,13 @@ def lib():
     L.wlg_upload.argtypes = [vp, dp]
     L.wlg_download.argtypes = [vp, dp]
     L.wlg_generate.argtypes = [vp, ctypes.c_uint64, ctypes.c_double]
-    L.wlg_hyp.argtypes = [vp, dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.wlg_ape_step.argtypes = [vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-    L.wlg_hyp_spatial_step.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int]
+    L.wlg_hyp.argtypes = [vp, dp] + [ctypes.c_int] * 5
+    L.wlg_ape_step.argtypes = [vp, ctypes.c_double] + [ctypes.c_int] * 4
+    L.wlg_hyp_spatial_step.argtypes = [vp, ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 3
+    L.wlg_set_bc.argtypes = [vp]
+    L.wlg_plaquette.argtypes = [vp, dp]
+    L.wlg_levels.argtypes = [vp, ctypes.POINTER(Parms), ctypes.c_int]
+    L.wlg_trim.argtypes = [vp]
     L.wlg_staple_sum.argtypes = [vp, dp, dp, ctypes.c_int, ctypes.c_int, dp]
     L.wlg_wloop.argtypes = [vp, ctypes.POINTER(Parms), ctypes.c_int, dp]
     L.wlg_measure.argtypes = [vp, ctypes.POINTER(Parms), ctypes.c_int, dp]
@@ -149,15 +153,32 @@ class Lattice:
         _check(lib().wlg_generate(self._h, seed, eps))
 
     # --- smearing -------------------------------------------------------
-    def hyp(self, alpha, proj_iter, time=True, space=True, precision=FP64):
+    def hyp(self, alpha, proj_iter, time=True, space=True, precision=FP64, proj=None):
         a = (ctypes.c_double * 3)(*alpha)
-        _check(lib().wlg_hyp(self._h, a, proj_iter, int(time), int(space), precision))
+        proj = int(proj_iter > 0) if proj is None else int(proj)
+        _check(lib().wlg_hyp(self._h, a, proj, proj_iter, int(time), int(space), precision))
 
-    def ape_step(self, alpha, proj_iter, four_d=False, precision=FP64):
-        _check(lib().wlg_ape_step(self._h, alpha, proj_iter, int(four_d), precision))
+    def ape_step(self, alpha, proj_iter, four_d=False, precision=FP64, proj=None):
+        proj = int(proj_iter > 0) if proj is None else int(proj)
+        _check(lib().wlg_ape_step(self._h, alpha, proj, proj_iter, int(four_d), precision))
 
-    def hyp_spatial_step(self, a1, a2, proj_iter, precision=FP64):
-        _check(lib().wlg_hyp_spatial_step(self._h, a1, a2, proj_iter, precision))
+    def hyp_spatial_step(self, a1, a2, proj_iter, precision=FP64, proj=None):
+        proj = int(proj_iter > 0) if proj is None else int(proj)
+        _check(lib().wlg_hyp_spatial_step(self._h, a1, a2, proj, proj_iter, precision))
+
+    def set_bc(self):
+        _check(lib().wlg_set_bc(self._h))
+
+    def plaquette(self):
+        out = ctypes.c_double()
+        _check(lib().wlg_plaquette(self._h, ctypes.byref(out)))
+        return out.value
+
+    def levels(self, parms, precision=FP64):
+        _check(lib().wlg_levels(self._h, ctypes.byref(parms), precision))
+
+    def trim(self):
+        _check(lib().wlg_trim(self._h))
 
     def staple_sum(self, a, b, mu, nu, v):
         v = np.ascontiguousarray(v, dtype=np.float64).copy()

@@ -48,6 +48,7 @@ struct wlg_ctx {
   Lat L;
   int bc;
   double2* U;         // gauge field, 4 slots FP64
+  double2* Uh;        // HYP-smeared copy made by wlg_measure (input stays intact)
   char* arena;        // scratch
   size_t arena_bytes;
   double* wl_dev;
@@ -137,6 +138,7 @@ extern "C" void wlg_destroy(wlg_ctx* c) {
   (void)hipStreamSynchronize(c->st);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->U) (void)hipFree(c->U);
+  if (c->Uh) (void)hipFree(c->Uh);
   if (c->arena) (void)hipFree(c->arena);
   if (c->wl_dev) (void)hipFree(c->wl_dev);
   if (c->partial) (void)hipFree(c->partial);
@@ -191,39 +193,43 @@ extern "C" int wlg_generate(wlg_ctx* c, uint64_t seed, double eps) {
   return 0;
 }
 
-static int run_hyp(wlg_ctx* c, const double a[3], int piter, int time, int space, int prec) {
-  if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
+// HYP of `in` into `out` (out may equal in: hyp_links smears udfld in place)
+static int run_hyp(wlg_ctx* c, const double2* in, double2* out, const double a[3], int proj,
+                   int piter, int time, int space, int prec) {
   size_t s = slot64(c->L);
+  if (out != in) CK(hipMemcpyAsync(out, in, 4 * s, hipMemcpyDeviceToDevice, c->st));
+  if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
   if (ensure(&c->arena, &c->arena_bytes, 24 * s)) return -1;
   double2* d1 = (double2*)c->arena;
   double2* d2 = (double2*)(c->arena + 12 * s);
-  int proj = piter > 0;
   if (prec == WLG_MIXED)
-    launch_hyp<float, double>(c->L, c->U, d1, d2, a, proj, piter, time, space, c->bc == 0, c->st);
+    launch_hyp<float, double>(c->L, in, out, d1, d2, a, proj, piter, time, space, c->bc == 0,
+                              c->st);
   else
-    launch_hyp<double, double>(c->L, c->U, d1, d2, a, proj, piter, time, space, c->bc == 0, c->st);
+    launch_hyp<double, double>(c->L, in, out, d1, d2, a, proj, piter, time, space, c->bc == 0,
+                               c->st);
   CKL();
   if (c->bc == 0 && !time) {
-    launch_zero_bc(c->L, c->U, c->st);  // set_bc (hyp_links.c:447)
+    launch_zero_bc(c->L, out, c->st);  // set_bc (hyp_links.c:447)
     CKL();
   }
   return 0;
 }
 
-extern "C" int wlg_hyp(wlg_ctx* c, const double alpha[3], int proj_iter, int time, int space,
-                       int prec) {
+extern "C" int wlg_hyp(wlg_ctx* c, const double alpha[3], int proj, int piter, int time,
+                       int space, int prec) {
   CK(hipSetDevice(c->device));
-  if (run_hyp(c, alpha, proj_iter, time, space, prec)) return -1;
+  if (run_hyp(c, c->U, c->U, alpha, proj, piter, time, space, prec)) return -1;
   CK(hipStreamSynchronize(c->st));
   return 0;
 }
 
-extern "C" int wlg_ape_step(wlg_ctx* c, double alpha, int proj_iter, int four_d, int prec) {
+extern "C" int wlg_ape_step(wlg_ctx* c, double alpha, int proj, int proj_iter, int four_d,
+                            int prec) {
   CK(hipSetDevice(c->device));
   size_t s = slot64(c->L);
   if (ensure(&c->arena, &c->arena_bytes, 4 * s)) return -1;
   double2* tmp = (double2*)c->arena;
-  int proj = proj_iter > 0;
   if (four_d) {
     launch_ape4(c->L, c->U, tmp, alpha, proj, proj_iter, c->st);
     CKL();
@@ -241,7 +247,8 @@ extern "C" int wlg_ape_step(wlg_ctx* c, double alpha, int proj_iter, int four_d,
   return 0;
 }
 
-extern "C" int wlg_hyp_spatial_step(wlg_ctx* c, double a1, double a2, int proj_iter, int prec) {
+extern "C" int wlg_hyp_spatial_step(wlg_ctx* c, double a1, double a2, int proj, int proj_iter,
+                                    int prec) {
   CK(hipSetDevice(c->device));
   size_t s = slot64(c->L);
   if (ensure(&c->arena, &c->arena_bytes, 9 * s)) return -1;
@@ -249,9 +256,9 @@ extern "C" int wlg_hyp_spatial_step(wlg_ctx* c, double a1, double a2, int proj_i
   double2* out = (double2*)(c->arena + 6 * s);
   const double2* F = c->U + 9 * c->L.V;
   if (prec == WLG_MIXED)
-    launch_hyp3<float, double>(c->L, F, D, out, a1, a2, proj_iter > 0, proj_iter, c->st);
+    launch_hyp3<float, double>(c->L, F, D, out, a1, a2, proj, proj_iter, c->st);
   else
-    launch_hyp3<double, double>(c->L, F, D, out, a1, a2, proj_iter > 0, proj_iter, c->st);
+    launch_hyp3<double, double>(c->L, F, D, out, a1, a2, proj, proj_iter, c->st);
   CKL();
   CK(hipMemcpyAsync(c->U + 9 * c->L.V, out, 3 * s, hipMemcpyDeviceToDevice, c->st));
   CK(hipStreamSynchronize(c->st));
@@ -344,7 +351,53 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  return 4 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p) * sizeof(double);
+  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p) * sizeof(double);
+}
+
+// operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
+// further APE (or 3-d HYP) steps per level, snapshot after each level.
+static int run_levels(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec, const Layout& o,
+                      const double2** lev) {
+  const Lat& L = c->L;
+  const int msl = p->msl;
+  const size_t lvl = (size_t)27 * L.V;
+  char* A = c->arena;
+  int proj = p->proj_iter > 0;
+  const double2* src = U + 9 * L.V;  // spatial links of the HYP-smeared field
+  double2* levbuf = (double2*)(A + o.lev);
+  double2* tmp0 = (double2*)(A + o.tmp);
+  double2* tmp1 = tmp0 + lvl;
+  double2* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
+  for (int sl = 0; sl < msl; sl++) {
+    const double2* cur = (sl == 0) ? src : lev[sl - 1];
+    int n = p->nss[sl];
+    if (n == 0) {
+      lev[sl] = cur;
+      continue;
+    }
+    for (int step = 0; step < n; step++) {
+      double2* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
+      if (p->op_smear == 1) {
+        if (prec == WLG_MIXED)
+          launch_hyp3<float, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
+                                     p->proj_iter, c->st);
+        else
+          launch_hyp3<double, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
+                                      p->proj_iter, c->st);
+      } else {
+        if (prec == WLG_MIXED)
+          launch_ape<float, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+        else
+          launch_ape<double, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+      }
+      CKL();
+      cur = dst;
+    }
+    lev[sl] = levbuf + sl * lvl;
+  }
+  for (int sl = 0; sl < msl; sl++) c->lev[sl] = lev[sl];
+  c->msl = msl;
+  return 0;
 }
 
 template <class TS, class TC>
@@ -383,7 +436,8 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
   return 0;
 }
 
-static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, hipEvent_t e_start) {
+static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec, double* wl_host,
+                     hipEvent_t e_start) {
   const Lat& L = c->L;
   if (check_parms(c, p)) return -1;
   const int msl = p->msl;
@@ -397,44 +451,9 @@ static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, 
   const size_t s64 = slot64(L), s32 = slot32(L);
   const size_t lvl = (size_t)27 * L.V;
   char* A = c->arena;
-  int proj = p->proj_iter > 0;
 
-  // ---- operator levels (smear.c:99-121): cumulative nss ----
-  const double2* src = c->U + 9 * L.V;  // spatial links of the HYP-smeared field
-  double2* levbuf = (double2*)(A + o.lev);
-  double2* tmp0 = (double2*)(A + o.tmp);
-  double2* tmp1 = tmp0 + lvl;
-  double2* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
   const double2* lev[5];
-  for (int sl = 0; sl < msl; sl++) {
-    const double2* cur = (sl == 0) ? src : lev[sl - 1];
-    int n = p->nss[sl];
-    if (n == 0) {
-      lev[sl] = cur;
-      continue;
-    }
-    for (int step = 0; step < n; step++) {
-      double2* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
-      if (p->op_smear == 1) {
-        if (prec == WLG_MIXED)
-          launch_hyp3<float, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
-                                     p->proj_iter, c->st);
-        else
-          launch_hyp3<double, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
-                                      p->proj_iter, c->st);
-      } else {
-        if (prec == WLG_MIXED)
-          launch_ape<float, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
-        else
-          launch_ape<double, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
-      }
-      CKL();
-      cur = dst;
-    }
-    lev[sl] = levbuf + sl * lvl;
-  }
-  for (int sl = 0; sl < msl; sl++) c->lev[sl] = lev[sl];
-  c->msl = msl;
+  if (run_levels(c, U, p, prec, o, lev)) return -1;
   hipEvent_t e_lev;
   if (record(c, &e_lev)) return -1;
 
@@ -447,7 +466,7 @@ static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, 
     float2* lev32 = (float2*)(A + o.lev32);
     const float2* lev32p[5];
     std::vector<const float2*> lts(p->mwlt);
-    launch_d2f((long)9 * L.V, c->U, lt32, c->st);  // L_1 = U_0 in FP32
+    launch_d2f((long)9 * L.V, U, lt32, c->st);  // L_1 = U_0 in FP32
     CKL();
     lts[0] = lt32;
     for (int T = 2; T <= p->mwlt; T++) {
@@ -456,10 +475,10 @@ static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, 
         launch_time_transporter<float, float, float, float>(L, lt32, 0, lts[T - 2], out32, nullptr,
                                                             c->st);
       } else if (T == kTimeFp64From) {
-        launch_time_transporter<double, double, float, double>(L, c->U, 0, lts[T - 2], ch[T % 2],
+        launch_time_transporter<double, double, float, double>(L, U, 0, lts[T - 2], ch[T % 2],
                                                                out32, c->st);
       } else {
-        launch_time_transporter<double, double, double, double>(L, c->U, 0, ch[(T - 1) % 2],
+        launch_time_transporter<double, double, double, double>(L, U, 0, ch[(T - 1) % 2],
                                                                 ch[T % 2], out32, c->st);
       }
       CKL();
@@ -479,11 +498,11 @@ static int run_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl_host, 
     (void)s32;
   } else {
     std::vector<const double2*> lts(p->mwlt);
-    lts[0] = c->U;  // slot 0 of the gauge field
+    lts[0] = U;  // slot 0 of the gauge field
     double2* ltb = (double2*)(A + o.lt);
     for (int T = 2; T <= p->mwlt; T++) {
       double2* out = ltb + (size_t)(T - 2) * 9 * L.V;
-      launch_time_transporter<double, double, double, double>(L, c->U, 0, lts[T - 2], out, nullptr,
+      launch_time_transporter<double, double, double, double>(L, U, 0, lts[T - 2], out, nullptr,
                                                               c->st);
       CKL();
       lts[T - 1] = out;
@@ -532,7 +551,7 @@ extern "C" int wlg_wloop(wlg_ctx* c, const wlg_parms* p, int prec, double* wl) {
   c->ev_used = 0;
   hipEvent_t e0;
   if (record(c, &e0)) return -1;
-  return run_wloop(c, p, prec, wl, e0);
+  return run_wloop(c, c->U, p, prec, wl, e0);
 }
 
 extern "C" int wlg_measure(wlg_ctx* c, const wlg_parms* p, int prec, double* wl) {
@@ -541,10 +560,13 @@ extern "C" int wlg_measure(wlg_ctx* c, const wlg_parms* p, int prec, double* wl)
   c->ev_used = 0;
   hipEvent_t e0, e1;
   if (record(c, &e0)) return -1;
+  if (!c->Uh) CK(hipMalloc((void**)&c->Uh, 4 * slot64(c->L)));
   int piter = p->proj_iter;
-  if (run_hyp(c, p->alpha_action, piter, 1, p->hyp_time_only ? 0 : 1, prec)) return -1;
+  if (run_hyp(c, c->U, c->Uh, p->alpha_action, piter > 0, piter, 1, p->hyp_time_only ? 0 : 1,
+              prec))
+    return -1;
   if (record(c, &e1)) return -1;
-  if (run_wloop(c, p, prec, wl, e1)) return -1;
+  if (run_wloop(c, c->Uh, p, prec, wl, e1)) return -1;
   float ms = 0.f;
   CK(hipEventElapsedTime(&ms, e0, e1));
   c->t_ms[WLG_T_HYP] += ms;
@@ -579,5 +601,55 @@ extern "C" double wlg_timer_ms(wlg_ctx* c, int stage) {
 extern "C" int wlg_contract_stats(wlg_ctx* c, long* launches, double* ms) {
   *launches = c->contract_launches;
   *ms = c->contract_ms;
+  return 0;
+}
+
+extern "C" int wlg_levels(wlg_ctx* c, const wlg_parms* p, int prec) {
+  CK(hipSetDevice(c->device));
+  if (check_parms(c, p)) return -1;
+  Layout o = wloop_layout(c->L, p, prec);
+  if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
+  const double2* lev[5];
+  if (run_levels(c, c->U, p, prec, o, lev)) return -1;
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_set_bc(wlg_ctx* c) {
+  CK(hipSetDevice(c->device));
+  if (c->bc == 0) {
+    launch_zero_bc(c->L, c->U, c->st);
+    CKL();
+  }
+  CK(hipStreamSynchronize(c->st));
+  return 0;
+}
+
+extern "C" int wlg_trim(wlg_ctx* c) {
+  CK(hipSetDevice(c->device));
+  CK(hipStreamSynchronize(c->st));
+  if (c->arena) CK(hipFree(c->arena));
+  if (c->partial) CK(hipFree(c->partial));
+  if (c->Uh) CK(hipFree(c->Uh));
+  c->arena = nullptr;
+  c->partial = nullptr;
+  c->Uh = nullptr;
+  c->arena_bytes = c->partial_bytes = 0;
+  c->msl = 0;
+  return 0;
+}
+
+extern "C" int wlg_plaquette(wlg_ctx* c, double* sum) {
+  CK(hipSetDevice(c->device));
+  int nblk = plaquette_blocks(c->L);
+  if (ensure(&c->arena, &c->arena_bytes, (size_t)nblk * sizeof(double))) return -1;
+  launch_plaquette(c->L, c->U, c->bc == 0, (double*)c->arena, nblk, c->st);
+  CKL();
+  std::vector<double> part(nblk);
+  CK(hipMemcpyAsync(part.data(), c->arena, nblk * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  CK(hipStreamSynchronize(c->st));
+  double s = 0.0;
+  for (int i = 0; i < nblk; i++) s += part[i];
+  *sum = s;
   return 0;
 }

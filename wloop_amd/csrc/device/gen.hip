@@ -195,6 +195,35 @@ __global__ __launch_bounds__(256) void d2f(long n, const double2* __restrict__ a
   if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
 }
 
+// Plaquette sum per workgroup (lnkplaq.c:48-128): per site the three spatial
+// planes and, unless x0 = N0-1 with open bc, the three time planes;
+// Re tr[(U_mu(x)U_nu(x+mu))^dag U_nu(x)U_mu(x+nu)].  Grid-stride loop over
+// sites, fixed-order reduction to one partial per workgroup.
+__global__ __launch_bounds__(256) void plaquette_kernel(Lat L, const double2* __restrict__ U,
+                                                        int open_bc, double* __restrict__ partial) {
+  double acc = 0.0;
+  for (long s = (long)blockIdx.x * 256 + threadIdx.x; s < L.V; s += (long)gridDim.x * 256) {
+    int x[4];
+    coords(L, s, x);
+    for (int mu = 1; mu < 4; mu++) {
+      for (int nu = 0; nu < mu; nu++) {
+        if (nu == 0 && open_bc && x[0] == L.n[0] - 1) continue;
+        M3<double> a, b, c, d;
+        load(a, U, L.V, mu, s);
+        load(b, U, L.V, nu, fwd(L, s, x, mu, 1));
+        load(c, U, L.V, nu, s);
+        load(d, U, L.V, mu, fwd(L, s, x, nu, 1));
+        acc += retr_dag_mul(mul(a, b), mul(c, d));
+      }
+    }
+  }
+  __shared__ double red[4];
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)((V + 255) / 256), ny, 1); }
 
 void launch_generate(const Lat& L, double2* U, double2* G, uint64_t seed, double eps, int open_bc,
@@ -214,6 +243,16 @@ void launch_soa_to_aos(long V, int nl, const double2* f, double* a, hipStream_t 
 
 void launch_zero_bc(const Lat& L, double2* U, hipStream_t st) {
   hipLaunchKernelGGL(zero_last_time_links, grid_for(L.st[0], 1), dim3(256), 0, st, L, U);
+}
+
+int plaquette_blocks(const Lat& L) {
+  long b = (L.V + 255) / 256;
+  return (int)(b < 1024 ? b : 1024);
+}
+
+void launch_plaquette(const Lat& L, const double2* U, int open_bc, double* partial, int nblk,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(plaquette_kernel, dim3(nblk), dim3(256), 0, st, L, U, open_bc, partial);
 }
 
 void launch_d2f(long n, const double2* a, float2* b, hipStream_t st) {

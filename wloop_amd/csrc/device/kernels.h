@@ -8,8 +8,9 @@ namespace wl {
 
 // smear.hip
 template <class TC, class TS>
-void launch_hyp(const Lat& L, v2<TS>* U, v2<TS>* D1, v2<TS>* D2, const double a[3], int proj,
-                int piter, int time, int space, int open_bc, hipStream_t st);
+void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* D2,
+                const double a[3], int proj, int piter, int time, int space, int open_bc,
+                hipStream_t st);
 template <class TC, class TS>
 void launch_hyp3(const Lat& L, const v2<TS>* F, v2<TS>* D, v2<TS>* out, double a1, double a2,
                  int proj, int piter, hipStream_t st);
@@ -28,6 +29,9 @@ void launch_aos_to_soa(long V, int nl, const double* a, double2* f, hipStream_t 
 void launch_soa_to_aos(long V, int nl, const double2* f, double* a, hipStream_t st);
 void launch_zero_bc(const Lat& L, double2* U, hipStream_t st);
 void launch_d2f(long n, const double2* a, float2* b, hipStream_t st);
+int plaquette_blocks(const Lat& L);
+void launch_plaquette(const Lat& L, const double2* U, int open_bc, double* partial, int nblk,
+                      hipStream_t st);
 
 // wloop.hip
 template <class TC, class TU, class TP, class TO>

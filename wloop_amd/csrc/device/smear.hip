@@ -105,10 +105,10 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
 }
 
 // Level 3: V_mu = P((1-a1)U_mu + a1/6 sum_{i=1..3} S_{mu nu}[V2[part{nu,mu}]]),
-// nu = (mu+i)&3.  Written in place into U (each thread reads only its own
-// U_mu(x) besides D2), blockIdx.y = mu - mu0.
+// nu = (mu+i)&3, written to out (which may be U itself: each thread reads
+// only its own U_mu(x) besides D2).  blockIdx.y = mu - mu0.
 template <class TC, class TS>
-__global__ __launch_bounds__(256) void hyp_level3(Lat L, v2<TS>* __restrict__ U,
+__global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>* out,
                                                   const v2<TS>* __restrict__ D2, int mu0,
                                                   double alpha, double beta, int proj,
                                                   int piter, int open_bc) {
@@ -124,11 +124,11 @@ __global__ __launch_bounds__(256) void hyp_level3(Lat L, v2<TS>* __restrict__ U,
     int n = part4(nu, mu);
     staple_add<TC, TS>(S, L, s, x, D2, 4 * n + nu, D2, 4 * n + mu, mu, nu);
   }
-  decorate_store<TC, TS>(S, U, mu, U, mu, L.V, s, alpha, beta, proj, piter);
+  decorate_store<TC, TS>(S, U, mu, out, mu, L.V, s, alpha, beta, proj, piter);
   if (open_bc && mu == 0 && x[0] == L.n[0] - 1) {
     M3<TS> z;
     set_zero(z);
-    store(z, U, L.V, 0, s);  // set_bc: U_0(x0 = N0-1) = 0
+    store(z, out, L.V, 0, s);  // set_bc: U_0(x0 = N0-1) = 0
   }
 }
 
@@ -236,16 +236,17 @@ __global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* _
 static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)((V + 255) / 256), ny, 1); }
 
 template <class TC, class TS>
-void launch_hyp(const Lat& L, v2<TS>* U, v2<TS>* D1, v2<TS>* D2, const double a[3], int proj,
-                int piter, int time, int space, int open_bc, hipStream_t st) {
+void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* D2,
+                const double a[3], int proj, int piter, int time, int space, int open_bc,
+                hipStream_t st) {
   hipLaunchKernelGGL((hyp_level1<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1,
                      1.0 - a[2], a[2] / 2.0, proj, piter);
   hipLaunchKernelGGL((hyp_level2<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1, D2,
                      1.0 - a[1], a[1] / 4.0, proj, piter);
   int mu0 = 1 - (time > 0), mu1 = 1 + 3 * (space > 0);
   if (mu1 > mu0)
-    hipLaunchKernelGGL((hyp_level3<TC, TS>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U, D2,
-                       mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
+    hipLaunchKernelGGL((hyp_level3<TC, TS>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U, out,
+                       D2, mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
 }
 
 template <class TC, class TS>
@@ -275,10 +276,10 @@ void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2
   hipLaunchKernelGGL(staple_sum_kernel, grid_for(L.V, 1), dim3(256), 0, st, L, A, B, v, mu, nu);
 }
 
-template void launch_hyp<double, double>(const Lat&, double2*, double2*, double2*, const double*,
-                                         int, int, int, int, int, hipStream_t);
-template void launch_hyp<float, double>(const Lat&, double2*, double2*, double2*, const double*,
-                                        int, int, int, int, int, hipStream_t);
+template void launch_hyp<double, double>(const Lat&, const double2*, double2*, double2*, double2*,
+                                         const double*, int, int, int, int, int, hipStream_t);
+template void launch_hyp<float, double>(const Lat&, const double2*, double2*, double2*, double2*,
+                                        const double*, int, int, int, int, int, hipStream_t);
 template void launch_hyp3<double, double>(const Lat&, const double2*, double2*, double2*, double,
                                           double, int, int, hipStream_t);
 template void launch_hyp3<float, double>(const Lat&, const double2*, double2*, double2*, double,
