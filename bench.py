@@ -114,6 +114,14 @@ def main():
     lat.synchronize()
     log("[bench] field generated on device")
 
+    from wloop_amd import autotune
+    tuned = {}
+    for prec, name in ((W.FP64, "fp64"), (W.MIXED, "mixed")):
+        ch = autotune.tune(lat, parms, prec, cache=None, log=log)
+        tuned[name] = {"kernel": "chain" if ch["kernel"] == W.KERNEL_CHAIN else "streaming",
+                       "wg": ch["wg"], "trial_ms_per_launch": ch["ms_per_launch"]}
+    log(f"[bench] autotuned contraction: {tuned}")
+
     w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
     log(f"[bench] fp64: {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
     wmx, tmx, latmx, timmx, csmx = time_path(lat, parms, W.MIXED, args.steps, args.warmup)
@@ -165,6 +173,7 @@ def main():
                  "stage_ms_per_config": {k: v / args.steps for k, v in tim64.items()},
                  "roofline": roof_64},
         "mixed_over_fp64": t64 / tmx,
+        "autotuned_contraction": tuned,
         "mixed_max_rel_err_vs_fp64": err,
         "stage_ms_per_config": {k: v / args.steps for k, v in timmx.items()},
         "roofline": roof_mx,

@@ -91,6 +91,19 @@ int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
 /* device bytes one measurement needs (admission by HBM footprint) */
 size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 
+/* launch-shape parameters of a context (0 = library default).  For each
+ * precision: the contraction kernel (1 streaming form, 2 chain form) and its
+ * workgroup parameter (chunks per (x0,k) for the streaming form, persistent
+ * workgroups for the chain form).  Results do not depend on them beyond
+ * floating-point summation order. */
+enum {
+   WLG_TUNE_FP64_CONTRACT_KERNEL = 0, WLG_TUNE_FP64_CONTRACT_WG,
+   WLG_TUNE_MIXED_CONTRACT_KERNEL, WLG_TUNE_MIXED_CONTRACT_WG,
+   WLG_TUNE_COUNT
+};
+int wlg_tune(wlg_ctx *ctx, int param, int value);
+int wlg_tune_get(wlg_ctx *ctx, int param);
+
 int wlg_synchronize(wlg_ctx *ctx);
 /* release the scratch memory of the context (the resident field stays) */
 int wlg_trim(wlg_ctx *ctx);

@@ -17,6 +17,8 @@ FP64, MIXED = 0, 1
 APE, HYP = 0, 1
 OPEN, PERIODIC = 0, 3
 T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
+TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG = range(4)
+KERNEL_STREAM, KERNEL_CHAIN = 1, 2
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
 
 
@@ -87,6 +89,8 @@ def lib():
     L.wlg_timer_ms.argtypes = [vp, ctypes.c_int]
     L.wlg_timer_ms.restype = ctypes.c_double
     L.wlg_contract_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long), dp]
+    L.wlg_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.wlg_tune_get.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
 
@@ -202,6 +206,12 @@ class Lattice:
         lev = np.empty((self.volume, 3, 18))
         _check(lib().wlg_level_download(self._h, sl, _dp(lev)))
         return lev
+
+    def tune(self, param, value):
+        _check(lib().wlg_tune(self._h, param, value))
+
+    def tune_get(self, param):
+        return lib().wlg_tune_get(self._h, param)
 
     def synchronize(self):
         _check(lib().wlg_synchronize(self._h))

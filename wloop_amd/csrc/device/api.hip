@@ -57,11 +57,13 @@ struct wlg_ctx {
   size_t partial_bytes;
   // operator levels of the last measurement (FP64, 3-slot fields)
   const double2* lev[5];
-  int msl;
+  const float2* lev32[5];
+  int msl, lev_fp32;
   // timing
   std::vector<hipEvent_t> ev;
   size_t ev_used;
   double t_ms[WLG_T_COUNT];
+  int tune[WLG_TUNE_COUNT];  // 0 = default
   long contract_launches;
   double contract_ms;
 };
@@ -311,23 +313,24 @@ static int check_parms(const wlg_ctx* c, const wlg_parms* p) {
 static const int kTimeFp64From = 9;
 
 struct Layout {
-  size_t lev, tmp, lt, lt32, lt64, s, lev32, total;
+  size_t lev, tmp, src32, lt, lt32, lt64, s, total;
 };
 
 static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
   size_t s64 = slot64(L), s32 = slot32(L);
   Layout o;
   size_t off = 0;
-  o.lev = off; off += (size_t)3 * p->msl * s64;
-  o.tmp = off; off += (p->op_smear == 1 ? 12 : 6) * s64;
+  const size_t sl = (prec == WLG_MIXED) ? s32 : s64;  // storage of levels and transporters
+  o.lev = off; off += (size_t)3 * p->msl * sl;
+  o.tmp = off; off += (p->op_smear == 1 ? 12 : 6) * sl;
   if (prec == WLG_MIXED) {
-    o.lev32 = off; off += (size_t)3 * p->msl * s32;
+    o.src32 = off; off += 3 * s32;
     o.lt32 = off; off += (size_t)p->mwlt * s32;
     o.lt64 = off; off += 2 * s64;
     o.lt = 0;
     o.s = off; off += (size_t)2 * 3 * p->msl * s32;
   } else {
-    o.lev32 = o.lt32 = o.lt64 = 0;
+    o.src32 = o.lt32 = o.lt64 = 0;
     o.lt = off; off += (size_t)(p->mwlt > 1 ? p->mwlt - 1 : 0) * s64;
     o.s = off; off += (size_t)2 * 3 * p->msl * s64;
   }
@@ -335,8 +338,16 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
   return o;
 }
 
-static size_t partial_count(const Lat& L, const wlg_parms* p) {
-  return (size_t)p->mwlr * p->mwlt * L.n[0] * 3 * contract_chunks(L) * p->msl * p->msl;
+// contraction variant: kernel form and its workgroup parameter
+static void contract_choice(const wlg_ctx* c, int prec, int* kernel, int* wg) {
+  const int* t = prec == WLG_MIXED ? c->tune + WLG_TUNE_MIXED_CONTRACT_KERNEL
+                                   : c->tune + WLG_TUNE_FP64_CONTRACT_KERNEL;
+  *kernel = t[0] > 0 ? t[0] - 1 : 0;
+  *wg = t[1] > 0 ? t[1] : (*kernel == 1 ? 512 : 8);
+}
+
+static size_t partial_count(const Lat& L, const wlg_parms* p, int kernel, int wg) {
+  return (size_t)p->mwlr * p->mwlt * L.n[0] * contract_slots(L, kernel, wg) * p->msl * p->msl;
 }
 
 extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
@@ -351,63 +362,74 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p) * sizeof(double);
+  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, 1, 2048) * sizeof(double);
 }
 
 // operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
 // further APE (or 3-d HYP) steps per level, snapshot after each level.
-static int run_levels(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec, const Layout& o,
-                      const double2** lev) {
+// TS = double: plain path (levels in FP64, staples in FP64).
+// TS = float: mixed path (levels stored in FP32, staples in FP32, the
+// projection after every step in FP64).  src is the 3-slot spatial field.
+template <class TS>
+static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, char* levbase,
+                        char* tmpbase, const v2<TS>** lev) {
   const Lat& L = c->L;
   const int msl = p->msl;
   const size_t lvl = (size_t)27 * L.V;
-  char* A = c->arena;
   int proj = p->proj_iter > 0;
-  const double2* src = U + 9 * L.V;  // spatial links of the HYP-smeared field
-  double2* levbuf = (double2*)(A + o.lev);
-  double2* tmp0 = (double2*)(A + o.tmp);
-  double2* tmp1 = tmp0 + lvl;
-  double2* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
+  v2<TS>* levbuf = (v2<TS>*)levbase;
+  v2<TS>* tmp0 = (v2<TS>*)tmpbase;
+  v2<TS>* tmp1 = tmp0 + lvl;
+  v2<TS>* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
   for (int sl = 0; sl < msl; sl++) {
-    const double2* cur = (sl == 0) ? src : lev[sl - 1];
+    const v2<TS>* cur = (sl == 0) ? src : lev[sl - 1];
     int n = p->nss[sl];
     if (n == 0) {
       lev[sl] = cur;
       continue;
     }
     for (int step = 0; step < n; step++) {
-      double2* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
-      if (p->op_smear == 1) {
-        if (prec == WLG_MIXED)
-          launch_hyp3<float, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
-                                     p->proj_iter, c->st);
-        else
-          launch_hyp3<double, double>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
-                                      p->proj_iter, c->st);
-      } else {
-        if (prec == WLG_MIXED)
-          launch_ape<float, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
-        else
-          launch_ape<double, double>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
-      }
+      v2<TS>* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
+      if (p->op_smear == 1)
+        launch_hyp3<TS, TS>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
+                            p->proj_iter, c->st);
+      else
+        launch_ape<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
       CKL();
       cur = dst;
     }
     lev[sl] = levbuf + sl * lvl;
   }
-  for (int sl = 0; sl < msl; sl++) c->lev[sl] = lev[sl];
-  c->msl = msl;
+  return 0;
+}
+
+static int run_levels(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec, const Layout& o,
+                      const double2** lev64, const float2** lev32) {
+  const Lat& L = c->L;
+  char* A = c->arena;
+  c->msl = p->msl;
+  c->lev_fp32 = (prec == WLG_MIXED);
+  if (prec == WLG_MIXED) {
+    float2* src = (float2*)(A + o.src32);
+    launch_d2f((long)27 * L.V, U + 9 * L.V, src, c->st);  // F_0 in FP32
+    CKL();
+    if (run_levels_t<float>(c, src, p, A + o.lev, A + o.tmp, lev32)) return -1;
+    for (int sl = 0; sl < p->msl; sl++) c->lev32[sl] = lev32[sl];
+  } else {
+    if (run_levels_t<double>(c, U + 9 * L.V, p, A + o.lev, A + o.tmp, lev64)) return -1;
+    for (int sl = 0; sl < p->msl; sl++) c->lev[sl] = lev64[sl];
+  }
   return 0;
 }
 
 template <class TS, class TC>
 static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* const* lev,
                                   const v2<TS>* const* LT, v2<TS>* sbuf0, v2<TS>* sbuf1,
-                                  int nchunk) {
+                                  int kernel, int wg) {
   const Lat& L = c->L;
   const int msl = p->msl;
   const size_t lvl = (size_t)27 * L.V;  // one 3-slot field in 2-vectors
-  const size_t pstride = (size_t)L.n[0] * 3 * nchunk * msl * msl;
+  const size_t pstride = (size_t)L.n[0] * contract_slots(L, kernel, wg) * msl * msl;
   const v2<TS>* S[5];
   const v2<TS>* prev[5];
   v2<TS>* bufs[2] = {sbuf0, sbuf1};
@@ -427,7 +449,7 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     if (record(c, &e1)) return -1;
     for (int t = 0; t < p->mwlt; t++) {
       launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S,
-                              c->partial + (size_t)(r * p->mwlt + t) * pstride, nchunk, c->st);
+                              c->partial + (size_t)(r * p->mwlt + t) * pstride, kernel, wg, c->st);
       CKL();
     }
     if (record(c, &e2)) return -1;
@@ -445,15 +467,17 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  const int nchunk = contract_chunks(L);
-  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p) * sizeof(double)))
+  int kernel, wg;
+  contract_choice(c, prec, &kernel, &wg);
+  const int nslot = contract_slots(L, kernel, wg);
+  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
     return -1;
   const size_t s64 = slot64(L), s32 = slot32(L);
-  const size_t lvl = (size_t)27 * L.V;
   char* A = c->arena;
 
   const double2* lev[5];
-  if (run_levels(c, U, p, prec, o, lev)) return -1;
+  const float2* lev32p[5];
+  if (run_levels(c, U, p, prec, o, lev, lev32p)) return -1;
   hipEvent_t e_lev;
   if (record(c, &e_lev)) return -1;
 
@@ -463,8 +487,6 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   if (prec == WLG_MIXED) {
     float2* lt32 = (float2*)(A + o.lt32);
     double2* ch[2] = {(double2*)(A + o.lt64), (double2*)(A + o.lt64 + s64)};
-    float2* lev32 = (float2*)(A + o.lev32);
-    const float2* lev32p[5];
     std::vector<const float2*> lts(p->mwlt);
     launch_d2f((long)9 * L.V, U, lt32, c->st);  // L_1 = U_0 in FP32
     CKL();
@@ -484,16 +506,11 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
       CKL();
       lts[T - 1] = out32;
     }
-    for (int sl = 0; sl < msl; sl++) {
-      launch_d2f(lvl, lev[sl], lev32 + sl * lvl, c->st);
-      CKL();
-      lev32p[sl] = lev32 + sl * lvl;
-    }
     if (record(c, &e_tt)) return -1;
     ev_before_rt = c->ev_used;
     float2* sb = (float2*)(A + o.s);
     if (run_transport_contract<float, float>(c, p, lev32p, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                             nchunk))
+                                             kernel, wg))
       return -1;
     (void)s32;
   } else {
@@ -511,14 +528,14 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     ev_before_rt = c->ev_used;
     double2* sb = (double2*)(A + o.s);
     if (run_transport_contract<double, double>(c, p, lev, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                               nchunk))
+                                               kernel, wg))
       return -1;
   }
   size_t ev_after_rt = c->ev_used;
 
   // ---- normalisation and wl_idx store (wloop.c:535, :584-597) ----
   double norm = 9.0 * L.n[1] * L.n[2] * L.n[3];
-  launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, nchunk, c->partial, norm, c->wl_dev, c->st);
+  launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, nslot, c->partial, norm, c->wl_dev, c->st);
   CKL();
   hipEvent_t e_end;
   if (record(c, &e_end)) return -1;
@@ -578,12 +595,26 @@ extern "C" int wlg_level_download(wlg_ctx* c, int sl, double* lev) {
   if (sl < 0 || sl >= c->msl) return set_err("level %d not available (msl=%d)", sl, c->msl);
   size_t bytes = (size_t)c->L.V * 3 * 18 * sizeof(double);
   double* stage = nullptr;
+  double2* wide = nullptr;
   CK(hipMalloc((void**)&stage, bytes));
-  launch_soa_to_aos(c->L.V, 3, c->lev[sl], stage, c->st);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipSuccess;
+  const double2* src = c->lev[sl];
+  if (c->lev_fp32) {
+    e = hipMalloc((void**)&wide, bytes);
+    if (e == hipSuccess) {
+      launch_f2d((long)27 * c->L.V, c->lev32[sl], wide, c->st);
+      e = hipGetLastError();
+    }
+    src = wide;
+  }
+  if (e == hipSuccess) {
+    launch_soa_to_aos(c->L.V, 3, src, stage, c->st);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipMemcpyAsync(lev, stage, bytes, hipMemcpyDeviceToHost, c->st);
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
   (void)hipFree(stage);
+  if (wide) (void)hipFree(wide);
   if (e != hipSuccess) return set_err("level download failed: %s", hipGetErrorString(e));
   return 0;
 }
@@ -610,7 +641,8 @@ extern "C" int wlg_levels(wlg_ctx* c, const wlg_parms* p, int prec) {
   Layout o = wloop_layout(c->L, p, prec);
   if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
   const double2* lev[5];
-  if (run_levels(c, c->U, p, prec, o, lev)) return -1;
+  const float2* lev32[5];
+  if (run_levels(c, c->U, p, prec, o, lev, lev32)) return -1;
   CK(hipStreamSynchronize(c->st));
   return 0;
 }
@@ -652,4 +684,18 @@ extern "C" int wlg_plaquette(wlg_ctx* c, double* sum) {
   for (int i = 0; i < nblk; i++) s += part[i];
   *sum = s;
   return 0;
+}
+
+extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
+  if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
+  if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
+  if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
+      value > 2)
+    return set_err("contraction kernel must be 0 (default), 1 (streaming) or 2 (chain)");
+  c->tune[param] = value;
+  return 0;
+}
+
+extern "C" int wlg_tune_get(wlg_ctx* c, int param) {
+  return (param >= 0 && param < WLG_TUNE_COUNT) ? c->tune[param] : -1;
 }

@@ -29,6 +29,7 @@ void launch_aos_to_soa(long V, int nl, const double* a, double2* f, hipStream_t 
 void launch_soa_to_aos(long V, int nl, const double2* f, double* a, hipStream_t st);
 void launch_zero_bc(const Lat& L, double2* U, hipStream_t st);
 void launch_d2f(long n, const double2* a, float2* b, hipStream_t st);
+void launch_f2d(long n, const float2* a, double2* b, hipStream_t st);
 int plaquette_blocks(const Lat& L);
 void launch_plaquette(const Lat& L, const double2* U, int open_bc, double* partial, int nblk,
                       hipStream_t st);
@@ -40,11 +41,12 @@ void launch_time_transporter(const Lat& L, const v2<TU>* U, int lu, const v2<TP>
 template <class TS>
 void launch_space_transporter(const Lat& L, int msl, const v2<TS>* const* F,
                               const v2<TS>* const* prev, v2<TS>* out, hipStream_t st);
-int contract_chunks(const Lat& L);
+int contract_slots(const Lat& L, int kernel, int wg);
 template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
-                     const v2<TS>* const* S, double* partial, int nchunk, hipStream_t st);
-void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nchunk, const double* partial,
+                     const v2<TS>* const* S, double* partial, int kernel, int wg,
+                     hipStream_t st);
+void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);
 
 }  // namespace wl

@@ -288,5 +288,9 @@ template void launch_ape<double, double>(const Lat&, const double2*, double2*, d
                                          hipStream_t);
 template void launch_ape<float, double>(const Lat&, const double2*, double2*, double, int, int,
                                         hipStream_t);
+template void launch_ape<float, float>(const Lat&, const float2*, float2*, double, int, int,
+                                       hipStream_t);
+template void launch_hyp3<float, float>(const Lat&, const float2*, float2*, float2*, double, double,
+                                        int, int, hipStream_t);
 
 }  // namespace wl

@@ -12,9 +12,9 @@
 // every level pair.  Each product has the same operands in the same order
 // as in the reference.
 //
-// Reduction is deterministic: wavefront butterfly, then the 4 waves of a
-// workgroup in fixed order into one partial per (R,T,x0,k,chunk,i,j); a
-// final kernel sums the partials in fixed order.
+// Reduction is deterministic: wavefront butterfly, per-wave accumulators in
+// LDS, the waves of a workgroup in fixed order into one partial per
+// (R,T,x0,workgroup,i,j); a final kernel sums the partials in fixed order.
 #include "su3.hpp"
 #include "kernels.h"
 
@@ -62,19 +62,27 @@ __global__ __launch_bounds__(256) void space_transporter(Lat L, LevelPtrs<TS> F,
   store(mul(f, p), out + (long)i * 27 * L.V, L.V, k, s);
 }
 
-// ---- contraction ----------------------------------------------------------
-// grid: x = chunk, y = k, z = x0.  Each workgroup strides over the spatial
-// slice and leaves MSL*MSL partial sums.
-template <int MSL, class TS, class TC, class TA>
-__global__ __launch_bounds__(256) void contract(Lat L, int R, int T, const v2<TS>* __restrict__ LT,
-                                                LevelPtrs<TS> S, TA* __restrict__ partial) {
-  const int k = blockIdx.y, x0 = blockIdx.z, nchunk = gridDim.x;
+// ---- contraction, streaming form ---------------------------------------
+// One thread per (site, k) of one source slice; it forms A_j = L_T(x) S_j(x+T0)
+// and B_i = S_i(x) L_T(x+Rk) and the msl^2 traces Re tr(A_j^dag B_i).
+// grid: x = chunk of the spatial slice, y = k, z = position of the source
+// slice along the chains x0 -> x0+T (mod N0), so that a slice read as sink
+// S_j(x+T0) is read as source S_i(x) by a workgroup dispatched next to it.
+// Partial sums: one per (x0, k, chunk, i, j).
+template <int MSL, class TS, class TC>
+__global__ __launch_bounds__(256) void contract_stream(Lat L, int R, int T, int g,
+                                                       const v2<TS>* __restrict__ LT,
+                                                       LevelPtrs<TS> S,
+                                                       double* __restrict__ partial) {
+  const int len = L.n[0] / g;
+  const int k = blockIdx.y, nchunk = gridDim.x;
+  const int x0 = (int)(((long)(blockIdx.z / len) + (long)(blockIdx.z % len) * T) % L.n[0]);
   const long slice = L.st[0];
   const long base = (long)x0 * slice;
   const long sbase = (long)((x0 + T) % L.n[0]) * slice;
-  TA acc[MSL * MSL];
+  double acc[MSL * MSL];
 #pragma unroll
-  for (int q = 0; q < MSL * MSL; q++) acc[q] = TA(0);
+  for (int q = 0; q < MSL * MSL; q++) acc[q] = 0.0;
 
   for (long xs = (long)blockIdx.x * 256 + threadIdx.x; xs < slice; xs += (long)nchunk * 256) {
     const long s = base + xs;
@@ -97,23 +105,23 @@ __global__ __launch_bounds__(256) void contract(Lat L, int R, int T, const v2<TS
       load(tmp, S.p[i], L.V, k, s);
       M3<TC> B = mul(convert<TC>(tmp), ltc);
 #pragma unroll
-      for (int j = 0; j < MSL; j++) acc[i * MSL + j] += (TA)retr_dag_mul(A[j], B);
+      for (int j = 0; j < MSL; j++) acc[i * MSL + j] += (double)retr_dag_mul(A[j], B);
     }
   }
 
   // wavefront butterfly, then the 4 waves in order
-  __shared__ TA red[4][MSL * MSL];
+  __shared__ double red[4][MSL * MSL];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < MSL * MSL; q++) {
-    TA v = acc[q];
+    double v = acc[q];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     if (lane == 0) red[wave][q] = v;
   }
   __syncthreads();
   if (threadIdx.x < MSL * MSL) {
-    TA v = red[0][threadIdx.x];
+    double v = red[0][threadIdx.x];
     v += red[1][threadIdx.x];
     v += red[2][threadIdx.x];
     v += red[3][threadIdx.x];
@@ -122,9 +130,121 @@ __global__ __launch_bounds__(256) void contract(Lat L, int R, int T, const v2<TS
   }
 }
 
-// Sum partials of one (R,T) block: out W_ij(R,T;x0) into wl (wl_idx order).
-// partial layout per (R,T): [x0][k][chunk][i*msl+j]
-__global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, int mwlr, int nchunk,
+// ---- contraction, chain form --------------------------------------------
+// W_ij(R,T;x0) = sum_{x,k} Re tr[S_j(x+T0)^dag D_i(x)],
+//   D_i(x) = L_T(x)^dag S_i(x) L_T(x+Rk)
+// which equals the reference's Re tr[(L_T(x) S_j(x+T0))^dag (S_i(x) L_T(x+Rk))].
+//
+// Chains: along x0 -> x0+T (mod N0) the sink links S_j(x0+T) of one step are
+// the source links of the next.  A thread owns one (site, k, level i) and
+// walks a chain of source slices c, c+T, c+2T, ...: it loads the msl sink
+// links once, takes msl traces against its carried D_i, then builds the
+// next D_i from its own sink link.  Every level link and time transporter is
+// read once per (R,T) launch (plus one extra slice per chain), instead of
+// twice.  Lanes of one (site,k) sit next to each other, so their identical
+// sink loads coalesce.
+//
+// Work units are (block of pairs, chain); a persistent workgroup takes units
+// w, w+nwg, ... in order.  Per step, the msl sums of each lane are reduced
+// over the pairs of the wave (butterfly) and added into a per-wave LDS
+// accumulator indexed by x0; at the end the waves are combined in fixed
+// order into one partial per (x0, workgroup, i, j).  Deterministic.
+template <int MSL, int MSLP, int WAVES, class TS, class TC>
+__global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g,
+                                                      const v2<TS>* __restrict__ LT,
+                                                      LevelPtrs<TS> S,
+                                                      double* __restrict__ partial) {
+  constexpr int PPW = 64 / MSLP;      // (site,k) pairs per wave
+  constexpr int PPB = PPW * WAVES;    // pairs per block unit
+  constexpr int MM = MSL * MSL;
+  extern __shared__ double lds_acc[];  // [WAVES][N0][MM]
+  const int N0 = L.n[0];
+  const int len = N0 / g;
+  const long slice = L.st[0];
+  const long npairs = 3 * slice;
+  const long nblocks = (npairs + PPB - 1) / PPB;
+  const long units = nblocks * g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = lane % MSLP;
+  const int pw = lane / MSLP;
+  double* acc_w = lds_acc + (long)wave * N0 * MM;
+
+  for (int q = lane; q < N0 * MM; q += 64) acc_w[q] = 0.0;
+
+  for (long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int c = (int)(u / nblocks);
+    const long b = u % nblocks;
+    const long pr = b * PPB + (long)wave * PPW + pw;
+    const bool live = (pw < PPW) && (i < MSL) && (pr < npairs);
+    const int k = live ? (int)(pr / slice) : 0;
+    const long xs = live ? pr % slice : 0;
+    int x[4];
+    coords(L, xs, x);  // x0 = 0 coordinates of the spatial site
+    M3<TC> D;
+    set_zero(D);
+    {
+      const long s0 = (long)c * slice + xs;
+      x[0] = c;
+      M3<TS> si, l0, lr;
+      if (live) {
+        load(si, S.p[i], L.V, k, s0);
+        load(l0, LT, L.V, 0, s0);
+        load(lr, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
+        D = mul(dag_mul(convert<TC>(l0), convert<TC>(si)), convert<TC>(lr));
+      }
+    }
+    int x0 = c;
+    for (int n = 0; n < len; n++) {
+      int x1 = x0 + T;
+      if (x1 >= N0) x1 -= N0;
+      const long s1 = (long)x1 * slice + xs;
+      double tr[MSL];
+      M3<TS> own;
+      set_zero(own);
+#pragma unroll
+      for (int j = 0; j < MSL; j++) {
+        M3<TS> sj;
+        if (live) load(sj, S.p[j], L.V, k, s1);
+        else set_zero(sj);
+        tr[j] = (double)retr_dag_mul(convert<TC>(sj), D);
+        if (j == i) own = sj;
+      }
+      // reduce over the pairs of the wave: lanes with equal i
+#pragma unroll
+      for (int j = 0; j < MSL; j++) {
+        double v = tr[j];
+#pragma unroll
+        for (int off = MSLP; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+        tr[j] = v;
+      }
+      if (lane < MSL) {
+#pragma unroll
+        for (int j = 0; j < MSL; j++) acc_w[x0 * MM + lane * MSL + j] += tr[j];
+      }
+      if (n + 1 < len && live) {
+        x[0] = x1;
+        M3<TS> l1, lr;
+        load(l1, LT, L.V, 0, s1);
+        load(lr, LT, L.V, 0, fwd(L, s1, x, k + 1, R));
+        D = mul(dag_mul(convert<TC>(l1), convert<TC>(own)), convert<TC>(lr));
+      }
+      x0 = x1;
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N0 * MM; q += blockDim.x) {
+    double v = lds_acc[q];
+#pragma unroll
+    for (int w = 1; w < WAVES; w++) v += lds_acc[(long)w * N0 * MM + q];
+    const int xo = q / MM, ij = q % MM;
+    partial[((long)xo * gridDim.x + blockIdx.x) * MM + ij] = v;
+  }
+}
+
+// W_ij(R,T;x0) from the partials of every (R,T) launch, in wl_idx order.
+// partial layout per (R,T): [x0][slot][i*msl+j] (nwg slots per x0); the slot
+// sums are added in fixed order with Neumaier compensation, then normalised.
+__global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg,
                                                     const double* __restrict__ partial,
                                                     double norm, double* __restrict__ wl) {
   long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,11 +258,15 @@ __global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, i
   int r = (int)(r_ % mwlr); r_ /= mwlr;
   int x0 = (int)r_;
   const int mm = msl * msl;
-  const double* p = partial + ((long)(r * mwlt + t) * N0 + x0) * 3 * nchunk * mm;
-  double sum = 0.0;
-  for (int k = 0; k < 3; k++)
-    for (int c = 0; c < nchunk; c++) sum += p[((long)k * nchunk + c) * mm + i * msl + j];
-  wl[id] = sum / norm;
+  const double* p = partial + (((long)(r * mwlt + t) * N0 + x0) * nwg) * mm + i * msl + j;
+  double sum = 0.0, comp = 0.0;
+  for (int w = 0; w < nwg; w++) {
+    double v = p[(long)w * mm];
+    double t2 = sum + v;
+    comp += (fabs(sum) >= fabs(v)) ? (sum - t2) + v : (v - t2) + sum;
+    sum = t2;
+  }
+  wl[id] = (sum + comp) / norm;
 }
 
 // ---- launchers ------------------------------------------------------------
@@ -168,36 +292,73 @@ void launch_space_transporter(const Lat& L, int msl, const v2<TS>* const* F,
                      out);
 }
 
-int contract_chunks(const Lat& L) {
-  long slice = L.st[0];
-  long maxc = (slice + 255) / 256;
-  long want = (2048 + 3L * L.n[0] - 1) / (3L * L.n[0]);
-  long c = want < maxc ? want : maxc;
-  return (int)(c < 1 ? 1 : c);
+int contract_slots(const Lat& L, int kernel, int wg) {
+  if (kernel == 1) return wg;
+  long maxc = (L.st[0] + 255) / 256;
+  return 3 * (int)(wg < maxc ? wg : maxc);
 }
 
-template <class TS, class TC>
-void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
-                     const v2<TS>* const* S, double* partial, int nchunk, hipStream_t st) {
-  LevelPtrs<TS> sp;
-  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
-  dim3 grid(nchunk, 3, L.n[0]);
-  switch (msl) {
-#define CASE(M)                                                                              \
-  case M:                                                                                    \
-    hipLaunchKernelGGL((contract<M, TS, TC, double>), grid, dim3(256), 0, st, L, R, T, LT, sp, \
-                       partial);                                                             \
-    break;
-    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5)
-#undef CASE
+template <int MSL, int MSLP, class TS, class TC>
+static void chain_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+                         const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st) {
+  constexpr int MM = MSL * MSL;
+  size_t lds4 = (size_t)4 * L.n[0] * MM * sizeof(double);
+  if (lds4 <= 65536) {
+    hipLaunchKernelGGL((contract_chain<MSL, MSLP, 4, TS, TC>), dim3(nwg), dim3(256), lds4, st, L, R,
+                       T, g, LT, sp, partial);
+  } else {
+    size_t lds1 = (size_t)L.n[0] * MM * sizeof(double);
+    hipLaunchKernelGGL((contract_chain<MSL, MSLP, 1, TS, TC>), dim3(nwg), dim3(64), lds1, st, L, R,
+                       T, g, LT, sp, partial);
   }
 }
 
-void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nchunk, const double* partial,
+template <int MSL, class TS, class TC>
+static void stream_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+                          const LevelPtrs<TS>& sp, double* partial, int nchunk, hipStream_t st) {
+  long maxc = (L.st[0] + 255) / 256;
+  if (nchunk > maxc) nchunk = (int)maxc;
+  hipLaunchKernelGGL((contract_stream<MSL, TS, TC>), dim3(nchunk, 3, L.n[0]), dim3(256), 0, st, L,
+                     R, T, g, LT, sp, partial);
+}
+
+// kernel 0: streaming form with wg chunks per (x0,k); 1: chain form with wg
+// persistent workgroups.  partial receives contract_slots() sums per x0.
+template <class TS, class TC>
+void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
+                     const v2<TS>* const* S, double* partial, int kernel, int wg, hipStream_t st) {
+  LevelPtrs<TS> sp;
+  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
+  int g = L.n[0], t = T % L.n[0];
+  while (t) {  // g = gcd(T, N0)
+    int r = g % t;
+    g = t;
+    t = r;
+  }
+  if (kernel == 1) {
+    switch (msl) {
+      case 1: chain_launch<1, 1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 2: chain_launch<2, 2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 3: chain_launch<3, 4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 4: chain_launch<4, 4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      default: chain_launch<5, 8, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+    }
+  } else {
+    switch (msl) {
+      case 1: stream_launch<1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 2: stream_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 3: stream_launch<3, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 4: stream_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      default: stream_launch<5, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+    }
+  }
+}
+
+void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st) {
   long nout = (long)msl * msl * mwlt * mwlr * N0;
-  hipLaunchKernelGGL(finish_wloop, grid_for(nout, 1), dim3(256), 0, st, N0, msl, mwlt, mwlr,
-                     nchunk, partial, norm, wl);
+  hipLaunchKernelGGL(finish_wloop, grid_for(nout, 1), dim3(256), 0, st, N0, msl, mwlt, mwlr, nwg,
+                     partial, norm, wl);
 }
 
 template void launch_time_transporter<double, double, double, double>(
@@ -211,8 +372,9 @@ template void launch_space_transporter<double>(const Lat&, int, const double2* c
 template void launch_space_transporter<float>(const Lat&, int, const float2* const*,
                                               const float2* const*, float2*, hipStream_t);
 template void launch_contract<double, double>(const Lat&, int, int, int, const double2*,
-                                              const double2* const*, double*, int, hipStream_t);
+                                              const double2* const*, double*, int, int,
+                                              hipStream_t);
 template void launch_contract<float, float>(const Lat&, int, int, int, const float2*,
-                                            const float2* const*, double*, int, hipStream_t);
+                                            const float2* const*, double*, int, int, hipStream_t);
 
 }  // namespace wl
