@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default launch shapes (for profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
@@ -117,6 +119,9 @@ def main():
     from wloop_amd import autotune
     tuned = {}
     for prec, name in ((W.FP64, "fp64"), (W.MIXED, "mixed")):
+        if args.no_autotune:
+            tuned[name] = "library default"
+            continue
         ch = autotune.tune(lat, parms, prec, cache=None, log=log)
         tuned[name] = {"kernel": "chain" if ch["kernel"] == W.KERNEL_CHAIN else "streaming",
                        "wg": ch["wg"], "trial_ms_per_launch": ch["ms_per_launch"]}
