@@ -47,6 +47,21 @@ void message(const char *format, ...);
 /* GPU device used by the module entry points (default 0) */
 void set_wloop_device(int device);
 
+/* configurations (openQCD archive.h names).  Exported format: 4 int32 lattice
+ * sizes, the average plaquette (double), then for every odd point x in
+ * lexicographic order (x3 fastest) the 8 links U(x,0),U(x-0,0),...,U(x,3),
+ * U(x-3,3), little endian.  import_cnfg checks sizes and plaquette. */
+void import_cnfg(const char *in);
+void export_cnfg(const char *out);
+/* lattice sizes from the header of an exported configuration */
+void cnfg_sizes(const char *in, int n[4]);
+/* No reference counterpart: a seeded synthetic configuration in udfld(),
+ * generated on the GPU (U = exp(i eps H), Haar gauge transform, set_bc) */
+void random_cnfg(unsigned long long seed, double eps);
+/* average plaquette Re tr / 3 over all plaquettes (open bc: time plaquettes
+ * at x0 = N0-1 left out) */
+double avg_plaq(void);
+
 #ifdef __cplusplus
 }
 #endif

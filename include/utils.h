@@ -32,6 +32,8 @@ typedef int32_t stdint_t;
 /* sections "[title]" and "tag value..." lines of the input read from stdin;
  * text after '#' is a comment; a tag must appear exactly once per section */
 long find_section(const char *title);
+/* like find_section, but returns -1 if the section is absent */
+long find_optional_section(const char *title);
 int read_line(const char *tag, const char *format, ...);
 void read_iprms(const char *tag, int n, int *iprms);
 void read_dprms(const char *tag, int n, double *dprms);

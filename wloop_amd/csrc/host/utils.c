@@ -57,6 +57,15 @@ static int is_section(const char *l, char *title, size_t n)
    return 1;
 }
 
+long find_optional_section(const char *title)
+{
+   load_input();
+   char t[256];
+   for (int i = 0; i < nlines; i++)
+      if (is_section(lines[i], t, sizeof(t)) && strcmp(t, title) == 0) return find_section(title);
+   return -1;
+}
+
 long find_section(const char *title)
 {
    load_input();
