@@ -88,6 +88,20 @@ int wlg_levels(wlg_ctx *ctx, const wlg_parms *p, int precision);
  * host AoS [V][3][18] */
 int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
 
+/* Batches (no reference counterpart): the full measurement (wlg_measure) of
+ * n contexts -- lattices of any sizes, one device -- captured once into one
+ * HIP graph whose per-context branches run concurrently.  wlg_batch_run
+ * replays it on the fields currently resident in the contexts (refresh them
+ * with wlg_generate/wlg_upload between runs) and copies W of context i to
+ * wl[i].  Buffers are allocated at creation; a context must not be measured
+ * eagerly with other parameters while it belongs to a batch. */
+typedef struct wlg_batch wlg_batch;
+int wlg_batch_create(wlg_batch **batch, int n, wlg_ctx *const *ctxs, const wlg_parms *parms,
+                     int precision);
+int wlg_batch_run(wlg_batch *batch, double *const *wl);
+double wlg_batch_ms(wlg_batch *batch);   /* device time of the last run */
+void wlg_batch_destroy(wlg_batch *batch);
+
 /* device bytes one measurement needs (admission by HBM footprint) */
 size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 

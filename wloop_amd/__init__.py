@@ -90,6 +90,13 @@ def lib():
     L.wlg_timer_ms.restype = ctypes.c_double
     L.wlg_contract_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long), dp]
     L.wlg_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.wlg_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp),
+                                   ctypes.POINTER(Parms), ctypes.c_int]
+    L.wlg_batch_run.argtypes = [vp, ctypes.POINTER(dp)]
+    L.wlg_batch_ms.argtypes = [vp]
+    L.wlg_batch_ms.restype = ctypes.c_double
+    L.wlg_batch_destroy.argtypes = [vp]
+    L.wlg_batch_destroy.restype = None
     L.wlg_tune_get.argtypes = [vp, ctypes.c_int]
     _lib = L
     return L
@@ -227,3 +234,39 @@ class Lattice:
         ms = ctypes.c_double()
         _check(lib().wlg_contract_stats(self._h, ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+
+class Batch:
+    """The measurement of several resident lattices (any sizes, one device)
+    captured once into one HIP graph (wlg_batch); run() replays it on the
+    fields currently resident in the lattices."""
+
+    def __init__(self, lattices, parms, precision=MIXED):
+        self.lattices = list(lattices)
+        self.parms = list(parms)
+        n = len(self.lattices)
+        handles = (ctypes.c_void_p * n)(*[l._h for l in self.lattices])
+        arr = (Parms * n)(*self.parms)
+        self._arr = arr
+        self._h = ctypes.c_void_p()
+        _check(lib().wlg_batch_create(ctypes.byref(self._h), n, handles, arr, precision))
+        self.out = [np.empty(wls(p, l.dims[0])) for l, p in zip(self.lattices, self.parms)]
+
+    def run(self):
+        ptrs = (ctypes.POINTER(ctypes.c_double) * len(self.out))(*[_dp(o) for o in self.out])
+        _check(lib().wlg_batch_run(self._h, ptrs))
+        return [o.copy() for o in self.out]
+
+    def device_ms(self):
+        return lib().wlg_batch_ms(self._h)
+
+    def close(self):
+        if self._h:
+            lib().wlg_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

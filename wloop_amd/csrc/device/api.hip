@@ -19,6 +19,7 @@
 using namespace wl;
 
 static thread_local char g_err[512] = "";
+static thread_local int g_capturing = 0;
 
 static int set_err(const char* fmt, ...) {
   va_list ap;
@@ -66,6 +67,11 @@ struct wlg_ctx {
   int tune[WLG_TUNE_COUNT];  // 0 = default
   long contract_launches;
   double contract_ms;
+  // graph capture: no host synchronisation, timers or allocation; W goes to
+  // the pinned buffer wl_pinned
+  int graph;
+  double* wl_pinned;
+  size_t wl_pinned_bytes;
 };
 
 extern "C" const char* wlg_error(void) { return g_err; }
@@ -75,6 +81,7 @@ static size_t slot32(const Lat& L) { return (size_t)9 * L.V * sizeof(float2); }
 
 static int ensure(char** p, size_t* have, size_t need) {
   if (*have >= need) return 0;
+  if (g_capturing) return set_err("device buffer of %zu bytes not prepared before graph capture", need);
   if (*p) CK(hipFree(*p));
   *p = nullptr;
   *have = 0;
@@ -93,6 +100,10 @@ static hipEvent_t next_event(wlg_ctx* c) {
 }
 
 static int record(wlg_ctx* c, hipEvent_t* e) {
+  if (c->graph) {
+    *e = nullptr;
+    return 0;
+  }
   *e = next_event(c);
   if (!*e) return set_err("hipEventCreate failed");
   CK(hipEventRecord(*e, c->st));
@@ -143,6 +154,7 @@ extern "C" void wlg_destroy(wlg_ctx* c) {
   if (c->Uh) (void)hipFree(c->Uh);
   if (c->arena) (void)hipFree(c->arena);
   if (c->wl_dev) (void)hipFree(c->wl_dev);
+  if (c->wl_pinned) (void)hipHostFree(c->wl_pinned);
   if (c->partial) (void)hipFree(c->partial);
   (void)hipStreamDestroy(c->st);
   delete c;
@@ -199,7 +211,10 @@ extern "C" int wlg_generate(wlg_ctx* c, uint64_t seed, double eps) {
 static int run_hyp(wlg_ctx* c, const double2* in, double2* out, const double a[3], int proj,
                    int piter, int time, int space, int prec) {
   size_t s = slot64(c->L);
-  if (out != in) CK(hipMemcpyAsync(out, in, 4 * s, hipMemcpyDeviceToDevice, c->st));
+  if (out != in) {
+    launch_copy((long)36 * c->L.V, in, out, c->st);
+    CKL();
+  }
   if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
   if (ensure(&c->arena, &c->arena_bytes, 24 * s)) return -1;
   double2* d1 = (double2*)c->arena;
@@ -537,6 +552,10 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   double norm = 9.0 * L.n[1] * L.n[2] * L.n[3];
   launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, nslot, c->partial, norm, c->wl_dev, c->st);
   CKL();
+  if (c->graph) {
+    CK(hipMemcpyAsync(c->wl_pinned, c->wl_dev, wls * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    return 0;
+  }
   hipEvent_t e_end;
   if (record(c, &e_end)) return -1;
   if (wl_host) CK(hipMemcpyAsync(wl_host, c->wl_dev, wls * sizeof(double), hipMemcpyDeviceToHost, c->st));
@@ -698,4 +717,147 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
 
 extern "C" int wlg_tune_get(wlg_ctx* c, int param) {
   return (param >= 0 && param < WLG_TUNE_COUNT) ? c->tune[param] : -1;
+}
+
+// ---- batches: several contexts measured by one captured graph ---------------
+
+// every buffer a measurement touches, allocated ahead of graph capture
+static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
+  if (check_parms(c, p)) return -1;
+  const Lat& L = c->L;
+  if (!c->Uh) CK(hipMalloc((void**)&c->Uh, 4 * slot64(L)));
+  Layout o = wloop_layout(L, p, prec);
+  size_t hyp = 24 * slot64(L);
+  if (ensure(&c->arena, &c->arena_bytes, o.total > hyp ? o.total : hyp)) return -1;
+  size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * L.n[0];
+  if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
+  int kernel, wg;
+  contract_choice(c, prec, &kernel, &wg);
+  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
+    return -1;
+  if (c->wl_pinned_bytes < wls * sizeof(double)) {
+    if (c->wl_pinned) CK(hipHostFree(c->wl_pinned));
+    c->wl_pinned = nullptr;
+    c->wl_pinned_bytes = 0;
+    CK(hipHostMalloc((void**)&c->wl_pinned, wls * sizeof(double), hipHostMallocDefault));
+    c->wl_pinned_bytes = wls * sizeof(double);
+  }
+  return 0;
+}
+
+struct wlg_batch {
+  int n, prec, device;
+  std::vector<wlg_ctx*> ctx;
+  std::vector<wlg_parms> parms;
+  hipStream_t st;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  hipEvent_t fork, t0, t1;
+  std::vector<hipEvent_t> join;
+  double last_ms;
+};
+
+static void batch_free(wlg_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  for (hipEvent_t e : b->join)
+    if (e) (void)hipEventDestroy(e);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+}
+
+extern "C" int wlg_batch_create(wlg_batch** out, int n, wlg_ctx* const* ctxs,
+                                const wlg_parms* parms, int prec) {
+  *out = nullptr;
+  if (n < 1) return set_err("a batch needs at least one context");
+  int dev = ctxs[0]->device;
+  for (int i = 0; i < n; i++) {
+    if (ctxs[i]->device != dev) return set_err("all contexts of a batch must be on one device");
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return set_err("context %d appears twice in the batch", i);
+  }
+  CK(hipSetDevice(dev));
+  for (int i = 0; i < n; i++)
+    if (prepare_measure(ctxs[i], &parms[i], prec)) return -1;
+  wlg_batch* b = new wlg_batch();
+  b->n = n;
+  b->prec = prec;
+  b->device = dev;
+  b->ctx.assign(ctxs, ctxs + n);
+  b->parms.assign(parms, parms + n);
+  b->join.assign(n, nullptr);
+  hipError_t e = hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreate(&b->t0);
+  if (e == hipSuccess) e = hipEventCreate(&b->t1);
+  for (int i = 0; i < n && e == hipSuccess; i++) e = hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming);
+  if (e != hipSuccess) {
+    set_err("batch setup failed: %s", hipGetErrorString(e));
+    batch_free(b);
+    return -1;
+  }
+  // capture: fork from the batch stream into every context's stream, run its
+  // measurement there, join back
+  e = hipStreamBeginCapture(b->st, hipStreamCaptureModeRelaxed);
+  if (e == hipSuccess) e = hipEventRecord(b->fork, b->st);
+  int rc = 0;
+  g_capturing = 1;
+  for (int i = 0; i < n && e == hipSuccess && rc == 0; i++) {
+    wlg_ctx* c = ctxs[i];
+    e = hipStreamWaitEvent(c->st, b->fork, 0);
+    if (e != hipSuccess) break;
+    c->graph = 1;
+    const wlg_parms* p = &parms[i];
+    rc = run_hyp(c, c->U, c->Uh, p->alpha_action, p->proj_iter > 0, p->proj_iter, 1,
+                 p->hyp_time_only ? 0 : 1, prec);
+    if (rc == 0) rc = run_wloop(c, c->Uh, p, prec, nullptr, nullptr);
+    c->graph = 0;
+    if (rc == 0) e = hipEventRecord(b->join[i], c->st);
+    if (rc == 0 && e == hipSuccess) e = hipStreamWaitEvent(b->st, b->join[i], 0);
+  }
+  g_capturing = 0;
+  hipError_t e2 = hipStreamEndCapture(b->st, &b->graph);
+  if (rc != 0 || e != hipSuccess || e2 != hipSuccess) {
+    if (rc == 0) set_err("graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    batch_free(b);
+    return -1;
+  }
+  e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    set_err("graph instantiation failed: %s", hipGetErrorString(e));
+    batch_free(b);
+    return -1;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int wlg_batch_run(wlg_batch* b, double* const* wl) {
+  CK(hipSetDevice(b->device));
+  CK(hipEventRecord(b->t0, b->st));
+  CK(hipGraphLaunch(b->exec, b->st));
+  CK(hipEventRecord(b->t1, b->st));
+  CK(hipStreamSynchronize(b->st));
+  float ms = 0.f;
+  CK(hipEventElapsedTime(&ms, b->t0, b->t1));
+  b->last_ms = ms;
+  for (int i = 0; i < b->n; i++) {
+    if (!wl || !wl[i]) continue;
+    const wlg_parms* p = &b->parms[i];
+    size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * b->ctx[i]->L.n[0];
+    memcpy(wl[i], b->ctx[i]->wl_pinned, wls * sizeof(double));
+  }
+  return 0;
+}
+
+extern "C" double wlg_batch_ms(wlg_batch* b) { return b->last_ms; }
+
+extern "C" void wlg_batch_destroy(wlg_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  (void)hipStreamSynchronize(b->st);
+  batch_free(b);
 }

@@ -224,6 +224,12 @@ __global__ __launch_bounds__(256) void plaquette_kernel(Lat L, const double2* __
   if (threadIdx.x == 0) partial[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+__global__ __launch_bounds__(256) void copy2(long n, const double2* __restrict__ a,
+                                             double2* __restrict__ b) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 __global__ __launch_bounds__(256) void f2d(long n, const float2* __restrict__ a,
                                            double2* __restrict__ b) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -259,6 +265,10 @@ int plaquette_blocks(const Lat& L) {
 void launch_plaquette(const Lat& L, const double2* U, int open_bc, double* partial, int nblk,
                       hipStream_t st) {
   hipLaunchKernelGGL(plaquette_kernel, dim3(nblk), dim3(256), 0, st, L, U, open_bc, partial);
+}
+
+void launch_copy(long n, const double2* a, double2* b, hipStream_t st) {
+  hipLaunchKernelGGL(copy2, grid_for(n, 1), dim3(256), 0, st, n, a, b);
 }
 
 void launch_f2d(long n, const float2* a, double2* b, hipStream_t st) {

@@ -30,6 +30,7 @@ void launch_soa_to_aos(long V, int nl, const double2* f, double* a, hipStream_t 
 void launch_zero_bc(const Lat& L, double2* U, hipStream_t st);
 void launch_d2f(long n, const double2* a, float2* b, hipStream_t st);
 void launch_f2d(long n, const float2* a, double2* b, hipStream_t st);
+void launch_copy(long n, const double2* a, double2* b, hipStream_t st);
 int plaquette_blocks(const Lat& L);
 void launch_plaquette(const Lat& L, const double2* U, int open_bc, double* partial, int nblk,
                       hipStream_t st);
