@@ -377,7 +377,7 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, 1, 2048) * sizeof(double);
+  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, 0, 32) * sizeof(double);
 }
 
 // operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
