@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default launch shapes (for profiling runs)")
+    ap.add_argument("--batch", type=int, default=2,
+                    help="configurations per captured graph in the batched leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
@@ -123,8 +125,9 @@ def main():
             tuned[name] = "library default"
             continue
         ch = autotune.tune(lat, parms, prec, cache=None, log=log)
-        tuned[name] = {"kernel": "chain" if ch["kernel"] == W.KERNEL_CHAIN else "streaming",
-                       "wg": ch["wg"], "trial_ms_per_launch": ch["ms_per_launch"]}
+        kname = {W.KERNEL_STREAM: "streaming", W.KERNEL_CHAIN: "chain (msl lanes per pair)",
+                 W.KERNEL_CHAIN1: "chain (one lane per pair)"}[ch["kernel"]]
+        tuned[name] = {"kernel": kname, "wg": ch["wg"], "trial_ms_per_launch": ch["ms_per_launch"]}
     log(f"[bench] autotuned contraction: {tuned}")
 
     w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
@@ -133,6 +136,40 @@ def main():
     log(f"[bench] mixed: {tmx / args.steps * 1e3:.1f} ms/config  stages {timmx}")
     err = rel_err(wmx, w64)
     log(f"[bench] mixed vs fp64 max rel err {err:.3e}")
+
+    batched = None
+    if args.batch > 0:
+        # mixed path, args.batch configurations of the workload per replay of one
+        # captured graph (fields regenerated on the device between replays,
+        # outside the timed replays)
+        lat.trim()
+        lats = [lat] + [W.Lattice(dims, cfg["bc"]) for _ in range(args.batch - 1)]
+        for l in lats[1:]:
+            autotune.apply(l, {"kernel": lat.tune_get(W.TUNE_MIXED_CONTRACT_KERNEL),
+                               "wg": lat.tune_get(W.TUNE_MIXED_CONTRACT_WG)}, W.MIXED)
+        batch = W.Batch(lats, [parms] * args.batch, W.MIXED)
+        bms = []
+        errs = []
+        for rep in range(args.warmup + args.steps):
+            for i, l in enumerate(lats):
+                l.generate(cfg["seed"] + 1000 * rep + i if i or rep else cfg["seed"], cfg["eps"])
+            t0 = time.perf_counter()
+            outs = batch.run()
+            dt = time.perf_counter() - t0
+            if rep >= args.warmup:
+                bms.append(dt)
+            if rep == 0:
+                errs.append(rel_err(outs[0], w64))
+        batch.close()
+        for l in lats[1:]:
+            l.close()
+        tb = sum(bms)
+        batched = {"configs_per_graph": args.batch, "value": args.batch * len(bms) / tb,
+                   "ms_per_config": tb / (args.batch * len(bms)) * 1e3,
+                   "latency_ms": {"p50": float(np.percentile(np.array(bms) * 1e3, 50)),
+                                  "p99": float(np.percentile(np.array(bms) * 1e3, 99))},
+                   "max_rel_err_vs_fp64": errs[0] if errs else None}
+        log(f"[bench] batched mixed: {batched}")
 
     V = int(np.prod(dims))
     links_per_site = 1 + 3 * parms.msl            # L_T + S_R of every level and direction
@@ -179,6 +216,7 @@ def main():
                  "roofline": roof_64},
         "mixed_over_fp64": t64 / tmx,
         "autotuned_contraction": tuned,
+        "batched_mixed": batched,
         "mixed_max_rel_err_vs_fp64": err,
         "stage_ms_per_config": {k: v / args.steps for k, v in timmx.items()},
         "roofline": roof_mx,

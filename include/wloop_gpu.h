@@ -106,9 +106,10 @@ void wlg_batch_destroy(wlg_batch *batch);
 size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 
 /* launch-shape parameters of a context (0 = library default).  For each
- * precision: the contraction kernel (1 streaming form, 2 chain form) and its
- * workgroup parameter (chunks per (x0,k) for the streaming form, persistent
- * workgroups for the chain form).  Results do not depend on them beyond
+ * precision: the contraction kernel (1 streaming form, 2 chain form with msl
+ * lanes per (site,k), 3 chain form with one lane per (site,k), msl <= 4) and
+ * its workgroup parameter (chunks per (x0,k) for the streaming form,
+ * persistent workgroups for the chain forms).  Results do not depend on them beyond
  * floating-point summation order. */
 enum {
    WLG_TUNE_FP64_CONTRACT_KERNEL = 0, WLG_TUNE_FP64_CONTRACT_WG,

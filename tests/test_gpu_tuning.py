@@ -19,7 +19,8 @@ def test_contraction_variants_agree(wl, precision):
     outs = []
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
-        for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512))):
+        for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
+                            (wl.KERNEL_CHAIN1, (1, 5, 256))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)

@@ -18,7 +18,7 @@ APE, HYP = 0, 1
 OPEN, PERIODIC = 0, 3
 T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
 TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG = range(4)
-KERNEL_STREAM, KERNEL_CHAIN = 1, 2
+KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1 = 1, 2, 3
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
 
 

@@ -14,7 +14,8 @@ import wloop_amd as W
 
 CANDIDATES = {
     W.KERNEL_STREAM: (4, 8, 16, 32),
-    W.KERNEL_CHAIN: (256, 512, 1024, 2048),
+    W.KERNEL_CHAIN: (512, 2048),
+    W.KERNEL_CHAIN1: (256, 512, 1024),
 }
 DEFAULT_CACHE = os.environ.get(
     "WLOOP_AMD_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "wloop_amd", "tune.json"))

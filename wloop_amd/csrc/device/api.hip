@@ -70,6 +70,7 @@ struct wlg_ctx {
   // graph capture: no host synchronisation, timers or allocation; W goes to
   // the pinned buffer wl_pinned
   int graph;
+  int cur_msl;  // msl of the measurement being prepared or run
   double* wl_pinned;
   size_t wl_pinned_bytes;
 };
@@ -353,12 +354,17 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
   return o;
 }
 
+static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
+
 // contraction variant: kernel form and its workgroup parameter
 static void contract_choice(const wlg_ctx* c, int prec, int* kernel, int* wg) {
   const int* t = prec == WLG_MIXED ? c->tune + WLG_TUNE_MIXED_CONTRACT_KERNEL
                                    : c->tune + WLG_TUNE_FP64_CONTRACT_KERNEL;
-  *kernel = t[0] > 0 ? t[0] - 1 : 0;
-  *wg = t[1] > 0 ? t[1] : (*kernel == 1 ? 512 : 8);
+  // defaults (measured at 48^3x96): streaming form for FP64, one-lane chain
+  // form for the FP32 transporters of the mixed path
+  *kernel = t[0] > 0 ? t[0] - 1 : (prec == WLG_MIXED ? 2 : 0);
+  if (*kernel == 2 && p_msl_gt4(c)) *kernel = 0;  // one-lane chain form needs msl <= 4
+  *wg = t[1] > 0 ? t[1] : (*kernel >= 1 ? 512 : 8);
 }
 
 static size_t partial_count(const Lat& L, const wlg_parms* p, int kernel, int wg) {
@@ -483,6 +489,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
   int kernel, wg;
+  c->cur_msl = p->msl;
   contract_choice(c, prec, &kernel, &wg);
   const int nslot = contract_slots(L, kernel, wg);
   if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
@@ -709,8 +716,9 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 2)
-    return set_err("contraction kernel must be 0 (default), 1 (streaming) or 2 (chain)");
+      value > 3)
+    return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
+                   "per pair) or 3 (chain, one lane per pair)");
   c->tune[param] = value;
   return 0;
 }
@@ -732,6 +740,7 @@ static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
   int kernel, wg;
+  c->cur_msl = p->msl;
   contract_choice(c, prec, &kernel, &wg);
   if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
     return -1;

@@ -241,6 +241,122 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
   }
 }
 
+// ---- contraction, one-lane chain form ----------------------------------
+// As the chain form, but one lane owns a (site, k) pair with the open loops
+// D_i of all msl levels, so every sink link is loaded by one lane only.  Per
+// step the msl^2 traces (computed in TC) of the 64 lanes of a wave are
+// transposed through LDS and summed in FP64 (16 lanes per value, then two
+// butterfly steps); the 4 wave sums are added in fixed order into a per-
+// workgroup FP64 accumulator indexed by x0 (one barrier per step: all waves
+// of a workgroup walk the same chain).  Requires msl <= 4.
+template <int MSL, class TS, class TC>
+__global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int g,
+                                                       const v2<TS>* __restrict__ LT,
+                                                       LevelPtrs<TS> S,
+                                                       double* __restrict__ partial) {
+  constexpr int MM = MSL * MSL;
+  static_assert(MM <= 16, "one-lane chain form needs msl <= 4");
+  extern __shared__ double lds_c1[];
+  const int N0 = L.n[0];
+  double* acc = lds_c1;                                   // [N0][MM]
+  double* red2 = lds_c1 + N0 * MM;                        // [2][4][16]
+  TC* redf = (TC*)(red2 + 2 * 4 * 16);                    // [4][64][16], traces in TC
+  const int len = N0 / g;
+  const long slice = L.st[0];
+  const long npairs = 3 * slice;
+  const long nblocks = (npairs + 255) / 256;
+  const long units = nblocks * g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  TC* rw = redf + wave * 64 * 16;
+
+  for (int q = threadIdx.x; q < N0 * MM; q += 256) acc[q] = 0.0;
+  __syncthreads();
+  int par = 0;
+  for (long u = blockIdx.x; u < units; u += gridDim.x) {
+    const int c = (int)(u / nblocks);
+    const long pr = (u % nblocks) * 256 + threadIdx.x;
+    const bool live = pr < npairs;
+    const int k = live ? (int)(pr / slice) : 0;
+    const long xs = live ? pr % slice : 0;
+    int x[4];
+    coords(L, xs, x);
+    M3<TC> D[MSL], Sx[MSL];
+    {
+      const long s0 = (long)c * slice + xs;
+      M3<TS> a, b, t;
+      load(a, LT, L.V, 0, s0);
+      load(b, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
+      M3<TC> ac = convert<TC>(a), bc = convert<TC>(b);
+#pragma unroll
+      for (int i = 0; i < MSL; i++) {
+        load(t, S.p[i], L.V, k, s0);
+        D[i] = mul(dag_mul(ac, convert<TC>(t)), bc);
+      }
+    }
+    int x0 = c;
+    int x1 = x0 + T;
+    if (x1 >= N0) x1 -= N0;
+#pragma unroll
+    for (int j = 0; j < MSL; j++) {
+      M3<TS> t;
+      load(t, S.p[j], L.V, k, (long)x1 * slice + xs);
+      Sx[j] = convert<TC>(t);
+    }
+    for (int n = 0; n < len; n++) {
+      const long s1 = (long)x1 * slice + xs;
+      M3<TS> l1, l1r;
+      if (n + 1 < len) {
+        load(l1, LT, L.V, 0, s1);
+        load(l1r, LT, L.V, 0, fwd(L, s1, x, k + 1, R));
+      }
+      // traces of this step, transposed through LDS, FP64 sums
+#pragma unroll
+      for (int i = 0; i < MSL; i++)
+#pragma unroll
+        for (int j = 0; j < MSL; j++)
+          rw[lane * 16 + i * MSL + j] = live ? retr_dag_mul(Sx[j], D[i]) : TC(0);
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const int q = lane & 15, grp = lane >> 4;
+      double sred = 0.0;
+      if (q < MM) {
+#pragma unroll
+        for (int m = 0; m < 16; m++) sred += (double)rw[(grp * 16 + m) * 16 + q];
+      }
+      sred += __shfl_xor(sred, 16, 64);
+      sred += __shfl_xor(sred, 32, 64);
+      if (lane < 16) red2[(par * 4 + wave) * 16 + lane] = sred;
+      __syncthreads();
+      if (threadIdx.x < MM) {
+        const double* r2 = red2 + par * 64;
+        acc[x0 * MM + threadIdx.x] += ((r2[threadIdx.x] + r2[16 + threadIdx.x]) + r2[32 + threadIdx.x]) +
+                                      r2[48 + threadIdx.x];
+      }
+      par ^= 1;
+      if (n + 1 < len) {
+        // next source slice x1: D_j = L(x1)^dag S_j(x1) L(x1+Rk)
+        M3<TC> a = convert<TC>(l1), b = convert<TC>(l1r);
+#pragma unroll
+        for (int j = 0; j < MSL; j++) D[j] = mul(dag_mul(a, Sx[j]), b);
+        x0 = x1;
+        x1 = x0 + T;
+        if (x1 >= N0) x1 -= N0;
+#pragma unroll
+        for (int j = 0; j < MSL; j++) {
+          M3<TS> t;
+          load(t, S.p[j], L.V, k, (long)x1 * slice + xs);
+          Sx[j] = convert<TC>(t);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N0 * MM; q += 256) {
+    const int xo = q / MM, ij = q % MM;
+    partial[((long)xo * gridDim.x + blockIdx.x) * MM + ij] = acc[q];
+  }
+}
+
 // W_ij(R,T;x0) from the partials of every (R,T) launch, in wl_idx order.
 // partial layout per (R,T): [x0][slot][i*msl+j] (nwg slots per x0); the slot
 // sums are added in fixed order with Neumaier compensation, then normalised.
@@ -293,7 +409,7 @@ void launch_space_transporter(const Lat& L, int msl, const v2<TS>* const* F,
 }
 
 int contract_slots(const Lat& L, int kernel, int wg) {
-  if (kernel == 1) return wg;
+  if (kernel >= 1) return wg;
   long maxc = (L.st[0] + 255) / 256;
   return 3 * (int)(wg < maxc ? wg : maxc);
 }
@@ -322,8 +438,19 @@ static void stream_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
                      R, T, g, LT, sp, partial);
 }
 
-// kernel 0: streaming form with wg chunks per (x0,k); 1: chain form with wg
-// persistent workgroups.  partial receives contract_slots() sums per x0.
+template <int MSL, class TS, class TC>
+static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st) {
+  size_t lds = (size_t)L.n[0] * MSL * MSL * sizeof(double) + 2 * 4 * 16 * sizeof(double) +
+               4 * 64 * 16 * sizeof(TC);
+  hipLaunchKernelGGL((contract_chain1<MSL, TS, TC>), dim3(nwg), dim3(256), lds, st, L, R, T, g, LT,
+                     sp, partial);
+}
+
+// kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl
+// lanes per pair) with wg persistent workgroups; 2: one-lane chain form with
+// wg persistent workgroups (msl <= 4; otherwise the streaming form).
+// partial receives contract_slots() sums per x0.
 template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                      const v2<TS>* const* S, double* partial, int kernel, int wg, hipStream_t st) {
@@ -335,7 +462,14 @@ void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
     g = t;
     t = r;
   }
-  if (kernel == 1) {
+  if (kernel == 2 && msl <= 4) {
+    switch (msl) {
+      case 1: chain1_launch<1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 2: chain1_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      case 3: chain1_launch<3, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+      default: chain1_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
+    }
+  } else if (kernel == 1) {
     switch (msl) {
       case 1: chain_launch<1, 1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
       case 2: chain_launch<2, 2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
