@@ -180,8 +180,9 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
 // One APE step on spatial links (ape_links.c:147-262 with time = 0):
 // out_mu = P(in_mu + alpha * sum_{eta != mu, spatial, ascending} S_{mu eta}[in,in]).
 // in and out are 3-slot spatial fields (slot k = mu-1); Jacobi update.
+// FP32 links: 4 waves per SIMD fit without spills (122 VGPRs)
 template <class TC, class TS>
-__global__ __launch_bounds__(256) void ape_step(Lat L, const v2<TS>* __restrict__ in,
+__global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : 1) void ape_step(Lat L, const v2<TS>* __restrict__ in,
                                                 v2<TS>* __restrict__ out, double alpha, int proj,
                                                 int piter) {
   long s = (long)blockIdx.x * blockDim.x + threadIdx.x;

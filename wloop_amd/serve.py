@@ -195,6 +195,14 @@ def run_stream(rate, n, precision, budget_gb, workers=2, sizes=SIZES, seed=0, lo
         if log and i % 10 == 0:
             log(f"[serve] submitted {i + 1}/{n}")
         time.sleep(rng.expovariate(rate))
+    while True:                         # progress while the queue drains
+        with srv.lock:
+            nd = len(srv.done)
+        if nd >= n:
+            break
+        if log:
+            log(f"[serve] completed {nd}/{n}")
+        time.sleep(5.0)
     srv.drain()
     out = summarize(reqs, t0)
     out.update({"offered_rate_per_s": rate, "precision": "mixed" if precision == W.MIXED else "fp64",
