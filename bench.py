@@ -27,6 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# FP64 vector peak; FP32 without packed math issues at the same rate
+VALU_PEAK_TFLOPS = 78.6
 
 WORKLOADS = {
     "c3": dict(dims=(96, 48, 48, 48), bc=0, eps=0.25, seed=3, nss=[0, 16, 32, 48], R=16, T=16,
@@ -181,19 +183,41 @@ def main():
     log("[bench] field generated on device")
 
     from wloop_amd import autotune
-    tuned = {}
+
+    def describe(ch):
+        d = {k: v for k, v in ch.items() if k not in ("trials", "cached")}
+        d["kernel"] = W.KERNEL_NAMES[ch["kernel"]]
+        return d
+
+    tuned, choice = {}, {}
     for prec, name in ((W.FP64, "fp64"), (W.MIXED, "mixed")):
         if args.no_autotune:
             tuned[name] = "library default"
             continue
-        ch = autotune.tune(lat, parms, prec, cache=None, log=log)
-        kname = {W.KERNEL_STREAM: "streaming", W.KERNEL_CHAIN: "chain (msl lanes per pair)",
-                 W.KERNEL_CHAIN1: "chain (one lane per pair)"}[ch["kernel"]]
-        tuned[name] = {"kernel": kname, "wg": ch["wg"], "trial_ms_per_launch": ch["ms_per_launch"]}
+        choice[name] = autotune.tune(lat, parms, prec, cache=None, log=log)
+        tuned[name] = describe(choice[name])
+    # the plain FP64 path keeps the reference's structure: one contraction
+    # launch per (R,T) over stored transporters (streaming or chain forms)
+    if not args.no_autotune:
+        plain = min((t for t in choice["fp64"]["trials"] if t["kernel"] != W.KERNEL_DEPTH),
+                    key=lambda t: t["ms_per_rt"])
+        tuned["fp64_plain"] = describe(plain)
+        autotune.apply(lat, plain, W.FP64)
     log(f"[bench] autotuned contraction: {tuned}")
 
     w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
-    log(f"[bench] fp64: {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
+    log(f"[bench] fp64 (plain): {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
+    best64 = None
+    if not args.no_autotune and choice["fp64"]["kernel"] == W.KERNEL_DEPTH:
+        autotune.apply(lat, choice["fp64"], W.FP64)
+        w64b, t64b, lat64b, tim64b, cs64b = time_path(lat, parms, W.FP64, args.steps, args.warmup)
+        log(f"[bench] fp64 (fused time slices): {t64b / args.steps * 1e3:.1f} ms/config  stages {tim64b}")
+        best64 = {"value": args.steps / t64b, "ms_per_step": t64b / args.steps * 1e3,
+                  "contraction": tuned["fp64"],
+                  "latency_ms": {"p50": float(np.percentile(lat64b, 50)),
+                                 "p99": float(np.percentile(lat64b, 99))},
+                  "stage_ms_per_config": {k: v / args.steps for k, v in tim64b.items()},
+                  "max_rel_err_vs_plain": rel_err(w64b, w64)}
     wmx, tmx, latmx, timmx, csmx = time_path(lat, parms, W.MIXED, args.steps, args.warmup)
     log(f"[bench] mixed: {tmx / args.steps * 1e3:.1f} ms/config  stages {timmx}")
     err = rel_err(wmx, w64)
@@ -207,8 +231,7 @@ def main():
         lat.trim()
         lats = [lat] + [W.Lattice(dims, cfg["bc"]) for _ in range(args.batch - 1)]
         for l in lats[1:]:
-            autotune.apply(l, {"kernel": lat.tune_get(W.TUNE_MIXED_CONTRACT_KERNEL),
-                               "wg": lat.tune_get(W.TUNE_MIXED_CONTRACT_WG)}, W.MIXED)
+            autotune.apply(l, autotune.current(lat, W.MIXED), W.MIXED)
         batch = W.Batch(lats, [parms] * args.batch, W.MIXED)
         bms = []
         errs = []
@@ -236,23 +259,40 @@ def main():
     V = int(np.prod(dims))
     links_per_site = 1 + 3 * parms.msl            # L_T + S_R of every level and direction
 
-    def roof(cs, link_bytes):
+    prof = os.path.join(ROOT, "profiles", "contract_traffic.json")
+    traffic = {}
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("bytes_per_launch", {})
+
+    def roof(cs, prec_name, kernel):
+        """roofline of the contraction launches of one path: HBM bytes for the
+        per-(R,T) forms, FLOPs for the fused time-slice form"""
         n, ms = cs
         avg_s = ms / n / 1e3
+        if kernel == W.KERNEL_DEPTH:
+            # per (x,k,T): 2 msl 3x3 products (198 flop) and msl^2 traces (36 flop)
+            flop = (args.steps * V * 3 * parms.mwlt * parms.mwlr *
+                    (2 * parms.msl * 198 + parms.msl ** 2 * 36) / n)
+            ach = flop / avg_s / 1e12
+            return {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / VALU_PEAK_TFLOPS, "traffic": traffic.get(f"{kernel}/{prec_name}"),
+                    "kernel": W.KERNEL_NAMES[kernel], "avg_launch_ms": avg_s * 1e3,
+                    "algorithmic_flop_per_launch": flop}
+        link_bytes = 72 if prec_name == "mixed" else 144
         algo = V * links_per_site * link_bytes
         ach = algo / avg_s / 1e9
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "contract",
-                "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes_per_launch": algo}
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(f"{kernel}/{prec_name}"),
+                "kernel": W.KERNEL_NAMES[kernel], "avg_launch_ms": avg_s * 1e3,
+                "algorithmic_bytes_per_launch": algo}
 
-    roof_mx = roof(csmx, 72)
-    roof_64 = roof(cs64, 144)
-    prof = os.path.join(ROOT, "profiles", "contract_traffic.json")
-    if os.path.exists(prof):
-        with open(prof) as f:
-            tr = json.load(f)
-        roof_mx["traffic"] = tr.get("mixed_bytes_per_launch")
-        roof_64["traffic"] = tr.get("fp64_bytes_per_launch")
+    k_mx = choice["mixed"]["kernel"] if choice else W.KERNEL_CHAIN1
+    k_64 = plain["kernel"] if choice else W.KERNEL_STREAM
+    roof_mx = roof(csmx, "mixed", k_mx)
+    roof_64 = roof(cs64, "fp64", k_64)
+    if best64:
+        best64["roofline"] = roof(cs64b, "fp64", W.KERNEL_DEPTH)
 
     out = {
         "metric": "configs/s at 48^3x96 (mixed vs FP64 GPU, rel err<1e-6); p50/p99 latency",
@@ -277,6 +317,8 @@ def main():
                  "stage_ms_per_config": {k: v / args.steps for k, v in tim64.items()},
                  "roofline": roof_64},
         "mixed_over_fp64": t64 / tmx,
+        "fp64_best": best64,
+        "mixed_over_fp64_best": (best64["ms_per_step"] / (tmx / args.steps * 1e3)) if best64 else t64 / tmx,
         "autotuned_contraction": tuned,
         "batched_mixed": batched,
         "mixed_max_rel_err_vs_fp64": err,

@@ -107,13 +107,18 @@ size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 
 /* launch-shape parameters of a context (0 = library default).  For each
  * precision: the contraction kernel (1 streaming form, 2 chain form with msl
- * lanes per (site,k), 3 chain form with one lane per (site,k), msl <= 4) and
- * its workgroup parameter (chunks per (x0,k) for the streaming form,
- * persistent workgroups for the chain forms).  Results do not depend on them beyond
- * floating-point summation order. */
+ * lanes per (site,k), 3 chain form with one lane per (site,k), msl <= 4,
+ * 4 fused time-slice form, msl <= 4) and its workgroup parameter (chunks per
+ * (x0,k) for the streaming form, persistent workgroups for the chain forms,
+ * workgroups per group of start slices for the fused form); for the fused
+ * form also the time-slice depth (T values per launch, 1..16) and the start
+ * slices per workgroup (4 or 8; one wave each).  Results do not depend on
+ * them beyond floating-point rounding and summation order. */
 enum {
    WLG_TUNE_FP64_CONTRACT_KERNEL = 0, WLG_TUNE_FP64_CONTRACT_WG,
    WLG_TUNE_MIXED_CONTRACT_KERNEL, WLG_TUNE_MIXED_CONTRACT_WG,
+   WLG_TUNE_FP64_CONTRACT_DEPTH, WLG_TUNE_FP64_CONTRACT_STARTS,
+   WLG_TUNE_MIXED_CONTRACT_DEPTH, WLG_TUNE_MIXED_CONTRACT_STARTS,
    WLG_TUNE_COUNT
 };
 int wlg_tune(wlg_ctx *ctx, int param, int value);

@@ -20,7 +20,7 @@ def test_contraction_variants_agree(wl, precision):
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
         for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
-                            (wl.KERNEL_CHAIN1, (1, 5, 256))):
+                            (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)
@@ -31,6 +31,46 @@ def test_contraction_variants_agree(wl, precision):
         assert rel_err(w, outs[0]) < (1e-13 if precision == 0 else 1e-7)
 
 
+# fused time-slice form: ragged pair blocks (spatial slice not a multiple of
+# 16), start groups that overrun N0, open and periodic boundaries, and on the
+# mixed path T blocks on both sides of the FP32/FP64 split at T = 9
+@pytest.mark.parametrize("dims,bc,mwlt,mwlr", [((10, 5, 6, 7), 0, 5, 2), ((20, 5, 6, 7), 3, 11, 3),
+                                               ((18, 4, 4, 6), 0, 10, 2)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision):
+    nss = [0, 2, 3]
+    u = O.generate(dims, bc, 45, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    pk, pw, pd, ps = ((wl.TUNE_MIXED_CONTRACT_KERNEL, wl.TUNE_MIXED_CONTRACT_WG, wl.TUNE_MIXED_CONTRACT_DEPTH,
+                       wl.TUNE_MIXED_CONTRACT_STARTS) if precision else
+                      (wl.TUNE_FP64_CONTRACT_KERNEL, wl.TUNE_FP64_CONTRACT_WG, wl.TUNE_FP64_CONTRACT_DEPTH,
+                       wl.TUNE_FP64_CONTRACT_STARTS))
+    tol = 1e-12 if precision == 0 else 1e-6
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        base = lat.measure(p, precision)  # library default form
+        assert rel_err(base, ref) < tol
+        lat.tune(pk, wl.KERNEL_DEPTH)
+        for depth, starts, wg in ((16, 4, 8), (3, 4, 5), (1, 8, 1), (8, 8, 16)):
+            lat.tune(pd, depth)
+            lat.tune(ps, starts)
+            lat.tune(pw, wg)
+            print(f"[fused] dims={dims} bc={bc} prec={precision} depth={depth} starts={starts} wg={wg}",
+                  flush=True)
+            w = lat.measure(p, precision)
+            assert rel_err(w, ref) < tol, (depth, starts, wg)
+            assert rel_err(w, base) < (1e-13 if precision == 0 else 1e-7), (depth, starts, wg)
+
+
+def test_fused_rejects_bad_shapes(wl):
+    with wl.Lattice((8, 4, 4, 4), 3) as lat:
+        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 5), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+                             (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
+            with pytest.raises(wl.WloopError):
+                lat.tune(param, value)
+
+
 def test_autotune_picks_a_candidate(wl, tmp_path):
     from wloop_amd import autotune
     dims = (8, 8, 8, 8)
@@ -38,9 +78,11 @@ def test_autotune_picks_a_candidate(wl, tmp_path):
     with wl.Lattice(dims, 3) as lat:
         lat.generate(1, 0.4)
         cache = str(tmp_path / "tune.json")
-        ch = autotune.tune(lat, p, wl.FP64, cache=cache, candidates={wl.KERNEL_STREAM: (2, 4),
-                                                                     wl.KERNEL_CHAIN: (64,)})
-        assert (ch["kernel"], ch["wg"]) in {(1, 2), (1, 4), (2, 64)}
+        cands = [{"kernel": wl.KERNEL_STREAM, "wg": 2}, {"kernel": wl.KERNEL_CHAIN, "wg": 64},
+                 {"kernel": wl.KERNEL_DEPTH, "wg": 8, "depth": 4, "starts": 4}]
+        ch = autotune.tune(lat, p, wl.FP64, cache=cache, candidates=cands)
+        assert (ch["kernel"], ch["wg"]) in {(1, 2), (2, 64), (4, 8)}
+        assert ch["ms_per_rt"] > 0
         assert lat.tune_get(wl.TUNE_FP64_CONTRACT_KERNEL) == ch["kernel"]
         again = autotune.tune(lat, p, wl.FP64, cache=cache)
         assert again["cached"] and again["kernel"] == ch["kernel"]

@@ -17,8 +17,12 @@ FP64, MIXED = 0, 1
 APE, HYP = 0, 1
 OPEN, PERIODIC = 0, 3
 T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
-TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG = range(4)
-KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1 = 1, 2, 3
+(TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG,
+ TUNE_FP64_CONTRACT_DEPTH, TUNE_FP64_CONTRACT_STARTS, TUNE_MIXED_CONTRACT_DEPTH,
+ TUNE_MIXED_CONTRACT_STARTS) = range(8)
+KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH = 1, 2, 3, 4
+KERNEL_NAMES = {KERNEL_STREAM: "streaming", KERNEL_CHAIN: "chain (msl lanes per pair)",
+                KERNEL_CHAIN1: "chain (one lane per pair)", KERNEL_DEPTH: "fused time slices"}
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
 
 

@@ -1,35 +1,45 @@
 """Launch-shape autotuner for the Wilson-loop contraction.
 
-For a lattice shape, boundary condition, number of operator levels and
-precision, it times every candidate (contraction form x workgroup parameter)
-on the resident field with HIP events and keeps the fastest; results are cached
-in a JSON file keyed by those properties, so a shape is tuned once per machine.
-The choice changes only the launch shape: results agree to floating-point
-summation order (checked in tests/test_gpu_tuning.py).
+For a lattice shape, boundary condition, number of operator levels, T range and
+precision, it times every candidate (contraction form x workgroup parameter,
+and for the fused time-slice form x time-slice depth x start slices per
+workgroup) on the resident field with HIP events and keeps the one with the
+least contraction time per (R,T) pair; results are cached in a JSON file keyed
+by those properties, so a shape is tuned once per machine.  The choice changes
+only the launch shape: results agree to floating-point rounding and summation
+order (checked in tests/test_gpu_tuning.py).
 """
 import json
 import os
 
 import wloop_amd as W
 
-CANDIDATES = {
-    W.KERNEL_STREAM: (4, 8, 16, 32),
-    W.KERNEL_CHAIN: (512, 2048),
-    W.KERNEL_CHAIN1: (256, 512, 1024),
-}
+
+def _cands():
+    out = [{"kernel": W.KERNEL_STREAM, "wg": wg} for wg in (4, 8, 16, 32)]
+    out += [{"kernel": W.KERNEL_CHAIN, "wg": wg} for wg in (512, 2048)]
+    out += [{"kernel": W.KERNEL_CHAIN1, "wg": wg} for wg in (256, 512, 1024)]
+    out += [{"kernel": W.KERNEL_DEPTH, "wg": wg, "depth": d, "starts": st}
+            for d in (8, 16) for st in (4, 8) for wg in (32, 64, 128)]
+    return out
+
+
+CANDIDATES = _cands()
 DEFAULT_CACHE = os.environ.get(
     "WLOOP_AMD_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "wloop_amd", "tune.json"))
 
 
 def _key(lat, parms, precision):
-    return "{}x{}x{}x{}-bc{}-msl{}-{}".format(*lat.dims, lat.bc, parms.msl,
-                                             "mixed" if precision == W.MIXED else "fp64")
+    return "{}x{}x{}x{}-bc{}-msl{}-T{}-{}".format(*lat.dims, lat.bc, parms.msl, parms.mwlt,
+                                                 "mixed" if precision == W.MIXED else "fp64")
 
 
 def _params(precision):
     if precision == W.MIXED:
-        return W.TUNE_MIXED_CONTRACT_KERNEL, W.TUNE_MIXED_CONTRACT_WG
-    return W.TUNE_FP64_CONTRACT_KERNEL, W.TUNE_FP64_CONTRACT_WG
+        return (W.TUNE_MIXED_CONTRACT_KERNEL, W.TUNE_MIXED_CONTRACT_WG, W.TUNE_MIXED_CONTRACT_DEPTH,
+                W.TUNE_MIXED_CONTRACT_STARTS)
+    return (W.TUNE_FP64_CONTRACT_KERNEL, W.TUNE_FP64_CONTRACT_WG, W.TUNE_FP64_CONTRACT_DEPTH,
+            W.TUNE_FP64_CONTRACT_STARTS)
 
 
 def _load(cache):
@@ -41,38 +51,47 @@ def _load(cache):
 
 
 def apply(lat, choice, precision):
-    pk, pw = _params(precision)
+    pk, pw, pd, ps = _params(precision)
     lat.tune(pk, choice["kernel"])
     lat.tune(pw, choice["wg"])
+    lat.tune(pd, choice.get("depth", 0))
+    lat.tune(ps, choice.get("starts", 0))
+
+
+def current(lat, precision):
+    """the launch shape a context uses now (0 = library default)"""
+    pk, pw, pd, ps = _params(precision)
+    return {"kernel": lat.tune_get(pk), "wg": lat.tune_get(pw), "depth": lat.tune_get(pd),
+            "starts": lat.tune_get(ps)}
 
 
 def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=False, log=None):
     """Pick and apply the fastest contraction launch shape; returns the choice
-    {"kernel", "wg", "ms_per_launch", "trials"}."""
+    {"kernel", "wg"[, "depth", "starts"], "ms_per_rt", "trials"}, ms_per_rt
+    being the contraction time per (R,T) pair."""
     key = _key(lat, parms, precision)
     db = _load(cache) if cache else {}
     if not force and key in db:
         apply(lat, db[key], precision)
         return dict(db[key], cached=True)
-    # a short measurement with the real level count: no APE steps, a few R,T
-    trial = W.make_parms([0] * parms.msl, min(parms.mwlt, 4), min(parms.mwlr, 4),
-                         proj_iter=parms.proj_iter)
+    # a short measurement with the real level count and T range: no APE steps,
+    # two values of R
+    trial = W.make_parms([0] * parms.msl, parms.mwlt, min(parms.mwlr, 2), proj_iter=parms.proj_iter)
     cands = candidates or CANDIDATES
     trials = []
-    for kernel, wgs in cands.items():
-        for wg in wgs:
-            apply(lat, {"kernel": kernel, "wg": wg}, precision)
-            lat.wloop(trial, precision)  # warm-up and allocation
-            lat.timers_reset()
-            for _ in range(2):
-                lat.wloop(trial, precision)
-            n, ms = lat.contract_stats()
-            trials.append({"kernel": kernel, "wg": wg, "ms_per_launch": ms / max(n, 1)})
-            if log:
-                log(f"[autotune] {key} kernel={kernel} wg={wg}: {ms / max(n, 1):.3f} ms/launch")
-    best = min(trials, key=lambda t: t["ms_per_launch"])
-    choice = {"kernel": best["kernel"], "wg": best["wg"], "ms_per_launch": best["ms_per_launch"],
-              "trials": trials}
+    for cand in cands:
+        apply(lat, cand, precision)
+        lat.wloop(trial, precision)  # warm-up and allocation
+        lat.timers_reset()
+        for _ in range(2):
+            lat.wloop(trial, precision)
+        _, ms = lat.contract_stats()
+        per_rt = ms / (2 * trial.mwlt * trial.mwlr)
+        trials.append(dict(cand, ms_per_rt=per_rt))
+        if log:
+            log(f"[autotune] {key} {cand}: {per_rt:.3f} ms per (R,T)")
+    best = min(trials, key=lambda t: t["ms_per_rt"])
+    choice = dict(best, trials=trials)
     apply(lat, choice, precision)
     if cache:
         db[key] = choice

@@ -332,7 +332,36 @@ struct Layout {
   size_t lev, tmp, src32, lt, lt32, lt64, s, total;
 };
 
-static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
+// contraction variant: kernel form, its workgroup parameter and, for the
+// fused time-slice form, the T depth per launch and start slices per workgroup
+struct Contract {
+  int kernel, wg, depth, starts;
+};
+
+// T blocks of the fused time-slice form: [Ta, Ta+d-1] covering 1..mwlt; on the
+// mixed path the blocks of T <= 8 run in FP32, those from T = 9 in FP64
+struct DepthBlock {
+  int Ta, d, fp64;
+};
+
+static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth) {
+  std::vector<DepthBlock> b;
+  const int split = prec == WLG_MIXED ? (mwlt < kTimeFp64From - 1 ? mwlt : kTimeFp64From - 1) : 0;
+  for (int T = 1; T <= split; T += depth) b.push_back({T, split - T + 1 < depth ? split - T + 1 : depth, 0});
+  for (int T = split + 1; T <= mwlt; T += depth)
+    b.push_back({T, mwlt - T + 1 < depth ? mwlt - T + 1 : depth, 1});
+  return b;
+}
+
+// FP64 time transporters kept for the starts of FP64 blocks on the mixed path
+static int depth_fp64_starts(int prec, int mwlt, int depth) {
+  if (prec != WLG_MIXED) return 0;
+  int n = 0;
+  for (const DepthBlock& b : depth_blocks(prec, mwlt, depth)) n += b.fp64;
+  return n;
+}
+
+static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64) {
   size_t s64 = slot64(L), s32 = slot32(L);
   Layout o;
   size_t off = 0;
@@ -342,7 +371,7 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
   if (prec == WLG_MIXED) {
     o.src32 = off; off += 3 * s32;
     o.lt32 = off; off += (size_t)p->mwlt * s32;
-    o.lt64 = off; off += 2 * s64;
+    o.lt64 = off; off += (size_t)(2 + nb64) * s64;
     o.lt = 0;
     o.s = off; off += (size_t)2 * 3 * p->msl * s32;
   } else {
@@ -356,19 +385,30 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec) {
 
 static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
 
-// contraction variant: kernel form and its workgroup parameter
-static void contract_choice(const wlg_ctx* c, int prec, int* kernel, int* wg) {
-  const int* t = prec == WLG_MIXED ? c->tune + WLG_TUNE_MIXED_CONTRACT_KERNEL
-                                   : c->tune + WLG_TUNE_FP64_CONTRACT_KERNEL;
+static Contract contract_choice(const wlg_ctx* c, int prec) {
+  const bool mixed = prec == WLG_MIXED;
+  const int* t = c->tune;
+  const int tk = t[mixed ? WLG_TUNE_MIXED_CONTRACT_KERNEL : WLG_TUNE_FP64_CONTRACT_KERNEL];
+  const int tw = t[mixed ? WLG_TUNE_MIXED_CONTRACT_WG : WLG_TUNE_FP64_CONTRACT_WG];
+  const int td = t[mixed ? WLG_TUNE_MIXED_CONTRACT_DEPTH : WLG_TUNE_FP64_CONTRACT_DEPTH];
+  const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
+  Contract k;
   // defaults (measured at 48^3x96): streaming form for FP64, one-lane chain
   // form for the FP32 transporters of the mixed path
-  *kernel = t[0] > 0 ? t[0] - 1 : (prec == WLG_MIXED ? 2 : 0);
-  if (*kernel == 2 && p_msl_gt4(c)) *kernel = 0;  // one-lane chain form needs msl <= 4
-  *wg = t[1] > 0 ? t[1] : (*kernel >= 1 ? 512 : 8);
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 0);
+  if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
+  k.wg = tw > 0 ? tw : (k.kernel == 3 ? 32 : k.kernel >= 1 ? 512 : 8);
+  k.depth = td > 0 ? td : 16;
+  k.starts = ts > 0 ? ts : 4;
+  // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
+  // time links on the FP64 blocks of the mixed path)
+  const int tsb = mixed ? 4 : 8;
+  while (k.starts > 4 && contract_depth_lds(c->cur_msl, k.starts, tsb, 8) > 163840) k.starts /= 2;
+  return k;
 }
 
-static size_t partial_count(const Lat& L, const wlg_parms* p, int kernel, int wg) {
-  return (size_t)p->mwlr * p->mwlt * L.n[0] * contract_slots(L, kernel, wg) * p->msl * p->msl;
+static size_t partial_count(const Lat& L, const wlg_parms* p, const Contract& k) {
+  return (size_t)p->mwlr * p->mwlt * L.n[0] * contract_slots(L, k.kernel, k.wg) * p->msl * p->msl;
 }
 
 extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
@@ -379,11 +419,14 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   L.st[1] = (long)dims[2] * dims[3];
   L.st[0] = (long)dims[1] * dims[2] * dims[3];
   L.V = (long)dims[0] * L.st[0];
-  Layout o = wloop_layout(L, p, prec);
+  // default launch shapes: FP64 time transporters of depth-16 blocks, and
+  // the partial sums of the streaming form with 32 chunks (the largest)
+  Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16));
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, 0, 32) * sizeof(double);
+  const Contract k = {0, 32, 16, 4};
+  return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, k) * sizeof(double);
 }
 
 // operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
@@ -443,14 +486,22 @@ static int run_levels(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec
   return 0;
 }
 
+// pointers of one fused-form block: L_Ta and the time links, in the
+// compute precision of the block
+struct DepthPtrs {
+  const void* lta;
+  const void* u0;
+};
+
 template <class TS, class TC>
 static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* const* lev,
                                   const v2<TS>* const* LT, v2<TS>* sbuf0, v2<TS>* sbuf1,
-                                  int kernel, int wg) {
+                                  const Contract& k, const std::vector<DepthBlock>& blocks,
+                                  const std::vector<DepthPtrs>& bptr) {
   const Lat& L = c->L;
   const int msl = p->msl;
   const size_t lvl = (size_t)27 * L.V;  // one 3-slot field in 2-vectors
-  const size_t pstride = (size_t)L.n[0] * contract_slots(L, kernel, wg) * msl * msl;
+  const size_t pstride = (size_t)L.n[0] * contract_slots(L, k.kernel, k.wg) * msl * msl;
   const v2<TS>* S[5];
   const v2<TS>* prev[5];
   v2<TS>* bufs[2] = {sbuf0, sbuf1};
@@ -468,13 +519,40 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
       for (int i = 0; i < msl; i++) S[i] = out + i * lvl;
     }
     if (record(c, &e1)) return -1;
-    for (int t = 0; t < p->mwlt; t++) {
-      launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S,
-                              c->partial + (size_t)(r * p->mwlt + t) * pstride, kernel, wg, c->st);
-      CKL();
+    if (k.kernel == 3) {
+      for (size_t q = 0; q < blocks.size(); q++) {
+        const DepthBlock& b = blocks[q];
+        double* part = c->partial + (size_t)(r * p->mwlt + b.Ta - 1) * pstride;
+        if constexpr (sizeof(TS) == 4) {
+          if (b.fp64)
+            launch_contract_depth<float, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
+                                                 (const double2*)bptr[q].lta,
+                                                 (const double2*)bptr[q].u0, S, part,
+                                                 (long)pstride, c->st);
+          else
+            launch_contract_depth<float, float>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
+                                                (const float2*)bptr[q].lta,
+                                                (const float2*)bptr[q].u0, S, part,
+                                                (long)pstride, c->st);
+        } else {
+          launch_contract_depth<double, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
+                                                (const double2*)bptr[q].lta,
+                                                (const double2*)bptr[q].u0, S, part,
+                                                (long)pstride, c->st);
+        }
+        CKL();
+      }
+      c->contract_launches += (long)blocks.size();
+    } else {
+      for (int t = 0; t < p->mwlt; t++) {
+        launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S,
+                                c->partial + (size_t)(r * p->mwlt + t) * pstride, k.kernel, k.wg,
+                                c->st);
+        CKL();
+      }
+      c->contract_launches += p->mwlt;
     }
     if (record(c, &e2)) return -1;
-    c->contract_launches += p->mwlt;
   }
   return 0;
 }
@@ -484,15 +562,18 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   const Lat& L = c->L;
   if (check_parms(c, p)) return -1;
   const int msl = p->msl;
-  Layout o = wloop_layout(L, p, prec);
+  c->cur_msl = p->msl;
+  const Contract k = contract_choice(c, prec);
+  std::vector<DepthBlock> blocks;
+  if (k.kernel == 3) blocks = depth_blocks(prec, p->mwlt, k.depth);
+  int nb64 = 0;
+  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
+  Layout o = wloop_layout(L, p, prec, nb64);
   if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  int kernel, wg;
-  c->cur_msl = p->msl;
-  contract_choice(c, prec, &kernel, &wg);
-  const int nslot = contract_slots(L, kernel, wg);
-  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
+  const int nslot = contract_slots(L, k.kernel, k.wg);
+  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
     return -1;
   const size_t s64 = slot64(L), s32 = slot32(L);
   char* A = c->arena;
@@ -503,54 +584,76 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   hipEvent_t e_lev;
   if (record(c, &e_lev)) return -1;
 
-  // ---- time transporters L_T for every T (wloop.c:153-204) ----
+  // ---- time transporters L_T (wloop.c:153-204) ----
+  // per-(R,T) forms: every T; fused form: the starts of its T blocks only
+  int tmax = p->mwlt;
+  if (k.kernel == 3) {
+    tmax = 1;
+    for (const DepthBlock& b : blocks) tmax = b.Ta > tmax ? b.Ta : tmax;
+  }
+  std::vector<DepthPtrs> bptr(blocks.size());
   size_t ev_before_rt = 0;
   hipEvent_t e_tt;
   if (prec == WLG_MIXED) {
     float2* lt32 = (float2*)(A + o.lt32);
     double2* ch[2] = {(double2*)(A + o.lt64), (double2*)(A + o.lt64 + s64)};
-    std::vector<const float2*> lts(p->mwlt);
+    double2* starts64 = (double2*)(A + o.lt64 + 2 * s64);  // FP64 L_Ta of the FP64 blocks
+    std::vector<const float2*> lts(p->mwlt, nullptr);
+    std::vector<const double2*> l64(p->mwlt + 1, nullptr);
     launch_d2f((long)9 * L.V, U, lt32, c->st);  // L_1 = U_0 in FP32
     CKL();
     lts[0] = lt32;
-    for (int T = 2; T <= p->mwlt; T++) {
+    int m64 = 0;
+    for (int T = 2; T <= tmax; T++) {
       float2* out32 = lt32 + (size_t)(T - 1) * 9 * L.V;
       if (T < kTimeFp64From) {
         launch_time_transporter<float, float, float, float>(L, lt32, 0, lts[T - 2], out32, nullptr,
                                                             c->st);
-      } else if (T == kTimeFp64From) {
-        launch_time_transporter<double, double, float, double>(L, U, 0, lts[T - 2], ch[T % 2],
-                                                               out32, c->st);
+        lts[T - 1] = out32;
       } else {
-        launch_time_transporter<double, double, double, double>(L, U, 0, ch[(T - 1) % 2],
-                                                                ch[T % 2], out32, c->st);
+        bool start = false;
+        for (const DepthBlock& b : blocks) start = start || (b.fp64 && b.Ta == T);
+        double2* dst = start ? starts64 + (size_t)(m64++) * 9 * L.V : ch[T % 2];
+        float2* o32 = k.kernel == 3 ? nullptr : out32;
+        if (T == kTimeFp64From)
+          launch_time_transporter<double, double, float, double>(L, U, 0, lts[T - 2], dst, o32,
+                                                                 c->st);
+        else
+          launch_time_transporter<double, double, double, double>(L, U, 0, l64[T - 1], dst, o32,
+                                                                  c->st);
+        l64[T] = dst;
+        if (o32) lts[T - 1] = out32;
       }
       CKL();
-      lts[T - 1] = out32;
+    }
+    for (size_t q = 0; q < blocks.size(); q++) {
+      if (blocks[q].fp64) bptr[q] = {l64[blocks[q].Ta], U};
+      else bptr[q] = {lts[blocks[q].Ta - 1], lt32};
     }
     if (record(c, &e_tt)) return -1;
     ev_before_rt = c->ev_used;
     float2* sb = (float2*)(A + o.s);
     if (run_transport_contract<float, float>(c, p, lev32p, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                             kernel, wg))
+                                             k, blocks, bptr))
       return -1;
     (void)s32;
   } else {
-    std::vector<const double2*> lts(p->mwlt);
+    std::vector<const double2*> lts(p->mwlt, nullptr);
     lts[0] = U;  // slot 0 of the gauge field
     double2* ltb = (double2*)(A + o.lt);
-    for (int T = 2; T <= p->mwlt; T++) {
+    for (int T = 2; T <= tmax; T++) {
       double2* out = ltb + (size_t)(T - 2) * 9 * L.V;
       launch_time_transporter<double, double, double, double>(L, U, 0, lts[T - 2], out, nullptr,
                                                               c->st);
       CKL();
       lts[T - 1] = out;
     }
+    for (size_t q = 0; q < blocks.size(); q++) bptr[q] = {lts[blocks[q].Ta - 1], U};
     if (record(c, &e_tt)) return -1;
     ev_before_rt = c->ev_used;
     double2* sb = (double2*)(A + o.s);
     if (run_transport_contract<double, double>(c, p, lev, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                               kernel, wg))
+                                               k, blocks, bptr))
       return -1;
   }
   size_t ev_after_rt = c->ev_used;
@@ -664,7 +767,7 @@ extern "C" int wlg_contract_stats(wlg_ctx* c, long* launches, double* ms) {
 extern "C" int wlg_levels(wlg_ctx* c, const wlg_parms* p, int prec) {
   CK(hipSetDevice(c->device));
   if (check_parms(c, p)) return -1;
-  Layout o = wloop_layout(c->L, p, prec);
+  Layout o = wloop_layout(c->L, p, prec, 0);
   if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
   const double2* lev[5];
   const float2* lev32[5];
@@ -716,9 +819,17 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 3)
+      value > 4)
     return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
-                   "per pair) or 3 (chain, one lane per pair)");
+                   "per pair), 3 (chain, one lane per pair) or 4 (fused time slices)");
+  if ((param == WLG_TUNE_FP64_CONTRACT_DEPTH || param == WLG_TUNE_MIXED_CONTRACT_DEPTH) &&
+      value > 16)
+    return set_err("time-slice depth must be 0 (default) or 1..16");
+  if ((param == WLG_TUNE_FP64_CONTRACT_STARTS || param == WLG_TUNE_MIXED_CONTRACT_STARTS) &&
+      value != 0 && value != 4 && value != 8)
+    return set_err("start slices per workgroup must be 0 (default), 4 or 8");
+  if ((param == WLG_TUNE_FP64_CONTRACT_WG || param == WLG_TUNE_MIXED_CONTRACT_WG) && value > 65536)
+    return set_err("workgroup parameter must be 0 (default) or 1..65536");
   c->tune[param] = value;
   return 0;
 }
@@ -734,15 +845,15 @@ static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
   if (check_parms(c, p)) return -1;
   const Lat& L = c->L;
   if (!c->Uh) CK(hipMalloc((void**)&c->Uh, 4 * slot64(L)));
-  Layout o = wloop_layout(L, p, prec);
+  c->cur_msl = p->msl;
+  const Contract k = contract_choice(c, prec);
+  const int nb64 = k.kernel == 3 ? depth_fp64_starts(prec, p->mwlt, k.depth) : 0;
+  Layout o = wloop_layout(L, p, prec, nb64);
   size_t hyp = 24 * slot64(L);
   if (ensure(&c->arena, &c->arena_bytes, o.total > hyp ? o.total : hyp)) return -1;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  int kernel, wg;
-  c->cur_msl = p->msl;
-  contract_choice(c, prec, &kernel, &wg);
-  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, kernel, wg) * sizeof(double)))
+  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
     return -1;
   if (c->wl_pinned_bytes < wls * sizeof(double)) {
     if (c->wl_pinned) CK(hipHostFree(c->wl_pinned));

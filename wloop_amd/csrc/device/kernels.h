@@ -47,6 +47,14 @@ template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                      const v2<TS>* const* S, double* partial, int kernel, int wg,
                      hipStream_t st);
+// fused time-slice form: T = Ta .. Ta+d-1 of one R, W start slices per
+// workgroup (one wave each), nslot workgroups per start group; partial points
+// at the block of T = Ta, pstride apart per T
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes);
+template <class TS, class TC>
+void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
+                           const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
+                           double* partial, long pstride, hipStream_t st);
 void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);
 

@@ -60,6 +60,14 @@ __device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, lon
 }
 
 // ---- products (element order as the textbook sum over j = 0,1,2) ----------
+// Each complex multiply-add is two fused multiply-adds per real component,
+// accumulated in the order of the sum over j (x*y - z*w written as one
+// expression costs a multiply, a fused multiply-add and an add).
+
+template <class R> __device__ __forceinline__ R fmadd(R a, R b, R c) { return __builtin_fma(a, b, c); }
+template <> __device__ __forceinline__ float fmadd<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
 
 // w = a * b
 template <class R> __device__ __forceinline__ M3<R> mul(const M3<R>& a, const M3<R>& b) {
@@ -68,12 +76,17 @@ template <class R> __device__ __forceinline__ M3<R> mul(const M3<R>& a, const M3
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      R re = a.re[3 * i] * b.re[k] - a.im[3 * i] * b.im[k];
-      R im = a.re[3 * i] * b.im[k] + a.im[3 * i] * b.re[k];
-      re += a.re[3 * i + 1] * b.re[3 + k] - a.im[3 * i + 1] * b.im[3 + k];
-      im += a.re[3 * i + 1] * b.im[3 + k] + a.im[3 * i + 1] * b.re[3 + k];
-      re += a.re[3 * i + 2] * b.re[6 + k] - a.im[3 * i + 2] * b.im[6 + k];
-      im += a.re[3 * i + 2] * b.im[6 + k] + a.im[3 * i + 2] * b.re[6 + k];
+      R re = a.re[3 * i] * b.re[k];
+      R im = a.re[3 * i] * b.im[k];
+      re = fmadd(-a.im[3 * i], b.im[k], re);
+      im = fmadd(a.im[3 * i], b.re[k], im);
+#pragma unroll
+      for (int j = 1; j < 3; j++) {
+        re = fmadd(a.re[3 * i + j], b.re[3 * j + k], re);
+        im = fmadd(a.re[3 * i + j], b.im[3 * j + k], im);
+        re = fmadd(-a.im[3 * i + j], b.im[3 * j + k], re);
+        im = fmadd(a.im[3 * i + j], b.re[3 * j + k], im);
+      }
       w.re[3 * i + k] = re;
       w.im[3 * i + k] = im;
     }
@@ -87,12 +100,17 @@ template <class R> __device__ __forceinline__ M3<R> mul_dag(const M3<R>& a, cons
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      R re = a.re[3 * i] * b.re[3 * k] + a.im[3 * i] * b.im[3 * k];
-      R im = a.im[3 * i] * b.re[3 * k] - a.re[3 * i] * b.im[3 * k];
-      re += a.re[3 * i + 1] * b.re[3 * k + 1] + a.im[3 * i + 1] * b.im[3 * k + 1];
-      im += a.im[3 * i + 1] * b.re[3 * k + 1] - a.re[3 * i + 1] * b.im[3 * k + 1];
-      re += a.re[3 * i + 2] * b.re[3 * k + 2] + a.im[3 * i + 2] * b.im[3 * k + 2];
-      im += a.im[3 * i + 2] * b.re[3 * k + 2] - a.re[3 * i + 2] * b.im[3 * k + 2];
+      R re = a.re[3 * i] * b.re[3 * k];
+      R im = a.im[3 * i] * b.re[3 * k];
+      re = fmadd(a.im[3 * i], b.im[3 * k], re);
+      im = fmadd(-a.re[3 * i], b.im[3 * k], im);
+#pragma unroll
+      for (int j = 1; j < 3; j++) {
+        re = fmadd(a.re[3 * i + j], b.re[3 * k + j], re);
+        im = fmadd(a.im[3 * i + j], b.re[3 * k + j], im);
+        re = fmadd(a.im[3 * i + j], b.im[3 * k + j], re);
+        im = fmadd(-a.re[3 * i + j], b.im[3 * k + j], im);
+      }
       w.re[3 * i + k] = re;
       w.im[3 * i + k] = im;
     }
@@ -106,12 +124,17 @@ template <class R> __device__ __forceinline__ M3<R> dag_mul(const M3<R>& a, cons
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      R re = a.re[i] * b.re[k] + a.im[i] * b.im[k];
-      R im = a.re[i] * b.im[k] - a.im[i] * b.re[k];
-      re += a.re[3 + i] * b.re[3 + k] + a.im[3 + i] * b.im[3 + k];
-      im += a.re[3 + i] * b.im[3 + k] - a.im[3 + i] * b.re[3 + k];
-      re += a.re[6 + i] * b.re[6 + k] + a.im[6 + i] * b.im[6 + k];
-      im += a.re[6 + i] * b.im[6 + k] - a.im[6 + i] * b.re[6 + k];
+      R re = a.re[i] * b.re[k];
+      R im = a.re[i] * b.im[k];
+      re = fmadd(a.im[i], b.im[k], re);
+      im = fmadd(-a.im[i], b.re[k], im);
+#pragma unroll
+      for (int j = 1; j < 3; j++) {
+        re = fmadd(a.re[3 * j + i], b.re[3 * j + k], re);
+        im = fmadd(a.re[3 * j + i], b.im[3 * j + k], im);
+        re = fmadd(a.im[3 * j + i], b.im[3 * j + k], re);
+        im = fmadd(-a.im[3 * j + i], b.re[3 * j + k], im);
+      }
       w.re[3 * i + k] = re;
       w.im[3 * i + k] = im;
     }
@@ -128,9 +151,13 @@ template <class R> __device__ __forceinline__ R retr_dag_mul(const M3<R>& a, con
   R r[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    R t = a.re[3 * i] * b.re[3 * i] + a.im[3 * i] * b.im[3 * i];
-    t += a.re[3 * i + 1] * b.re[3 * i + 1] + a.im[3 * i + 1] * b.im[3 * i + 1];
-    t += a.re[3 * i + 2] * b.re[3 * i + 2] + a.im[3 * i + 2] * b.im[3 * i + 2];
+    R t = a.re[3 * i] * b.re[3 * i];
+    t = fmadd(a.im[3 * i], b.im[3 * i], t);
+#pragma unroll
+    for (int j = 1; j < 3; j++) {
+      t = fmadd(a.re[3 * i + j], b.re[3 * i + j], t);
+      t = fmadd(a.im[3 * i + j], b.im[3 * i + j], t);
+    }
     r[i] = t;
   }
   return r[0] + r[1] + r[2];

@@ -10,7 +10,7 @@
 // A_j = L_T(x) S_j(x+T0) and B_i = S_i(x) L_T(x+Rk) once (2 msl products)
 // and takes the msl^2 traces; the reference recomputes both products for
 // every level pair.  Each product has the same operands in the same order
-// as in the reference.
+// as in the reference (with fused multiply-adds).
 //
 // Reduction is deterministic: wavefront butterfly, per-wave accumulators in
 // LDS, the waves of a workgroup in fixed order into one partial per
@@ -357,6 +357,253 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
   }
 }
 
+// ---- contraction, fused time-slice form ----------------------------------
+// One launch covers T = Ta .. Ta+d-1 of one R.  A lane owns (pair p of a block
+// of 16 (site,k) pairs, level i) for one start slice x0 and carries
+//     D_i(T) = L_T(x)^dag S_i(x) L_T(x+Rk)
+// from T = Ta (built from the stored L_Ta) to Ta+d-1 through
+//     D_i(T+1) = U_0(x+T0)^dag D_i(T) U_0(x+Rk+T0),
+// i.e. L_{T+1}(x) = L_T(x) U_0(x+T0), and takes Re tr[S_j(x+T0)^dag D_i(T)]
+// for every level j at every T (the reference's trace, wloop.c:436-508).
+// The transporters of the intermediate T are never stored.
+//
+// The W waves of a workgroup own the start slices x0g .. x0g+W-1 of the same
+// pair block.  At step n wave w needs, from time slice x0g+Ta+n+w, the sink
+// links S_j of the block and the time links at x and x+Rk; a ring of W+1
+// LDS slots holds the W slices of the step while the slice of the next step
+// is fetched into registers and stored into the free slot.  Level and time
+// links are fetched from HBM about (W+d-1)/(W d) times per (x0,T) instead of
+// once, and no global load is waited for inside a step.
+//
+// Lane layout: lane = 16 i + p (a row of 16 lanes per level).  The traces of a
+// step are summed over the 16 pairs of a row in FP64 by a butterfly that
+// hands half of the carried values to the partner in each of its first two
+// stages, then added into a per-wave LDS accumulator acc[T][i][j]; one
+// partial per (T, x0, workgroup slot, i, j) is stored at the end.  Workgroup
+// b serves start group b / nslot and pair blocks b % nslot + m nslot, so the
+// workgroups that read the same pair blocks sit on the same XCD when nslot is
+// a multiple of 8.  Deterministic.  Requires msl <= 4, d <= 16.
+// waves per SIMD the LDS of the form allows at W = 4 (FP64 time links) or
+// W = 4 / 8 (FP32 time links), as register target
+template <int MSL, int W, class TS, class TC>
+__global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) void contract_depth(Lat L, int R, int Ta, int d, int nslot,
+                                                         const v2<TC>* __restrict__ LTa,
+                                                         const v2<TC>* __restrict__ U0,
+                                                         LevelPtrs<TS> S,
+                                                         double* __restrict__ partial,
+                                                         long pstride) {
+  static_assert(MSL <= 4, "fused time-slice form needs msl <= 4");
+  constexpr int MM = MSL * MSL;
+  constexpr int NT = 64 * W;
+  constexpr int NS = MSL * 9 * 16;             // sink 2-vectors of a slot: [j][e][p]
+  constexpr int NU = 2 * 9 * 16;               // time-link 2-vectors: [x / x+Rk][e][p]
+  constexpr int PS = (NS + NT - 1) / NT, PU = (NU + NT - 1) / NT;
+  constexpr int SLOT = NS * (int)sizeof(v2<TS>) + NU * (int)sizeof(v2<TC>);  // bytes
+  extern __shared__ double lds_d[];
+  double* acc = lds_d;                                   // [W][16][16]
+  long* xr_tab = (long*)(lds_d + W * 256);               // [16] x+Rk of the block's pairs
+  char* ring = (char*)(xr_tab + 16);                     // [W+1] slots
+  const int N0 = L.n[0];
+  const long slice = L.st[0];
+  const long V = L.V;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = lane >> 4, p = lane & 15;
+  const int jw = blockIdx.x % nslot;
+  const int x0g = (blockIdx.x / nslot) * W;
+  const int x0 = x0g + w;
+  const bool wave_live = x0 < N0;
+  const long nbk = (slice + 15) / 16;
+  const long nb = 3 * nbk;
+
+  // this thread's elements of a ring slot: sink element q = (j*9 + e)*16 + pp
+  // and time-link element q = (h*9 + e)*16 + pp (h = 1: the site x + R k)
+  int sj_[PS], sp_[PS], uh_[PU], up_[PU];
+  long so_[PS], uo_[PU];
+#pragma unroll
+  for (int m = 0; m < PS; m++) {
+    const int q = threadIdx.x + m * NT;
+    sj_[m] = q < NS ? q / 144 : -1;
+    sp_[m] = q & 15;
+    so_[m] = (long)((q >> 4) % 9) * V;
+  }
+#pragma unroll
+  for (int m = 0; m < PU; m++) {
+    const int q = threadIdx.x + m * NT;
+    uh_[m] = q < NU ? q / 144 : -1;
+    up_[m] = q & 15;
+    uo_[m] = (long)((q >> 4) % 9) * V;
+  }
+
+  for (int q = threadIdx.x; q < W * 256; q += NT) acc[q] = 0.0;
+
+  for (long bb = jw; bb < nb; bb += nslot) {
+    const int k = (int)(bb / nbk);
+    const long xs0 = (bb % nbk) * 16;
+    const long xs = xs0 + p;
+    const bool pair_live = xs < slice;
+    const bool live = wave_live && i < MSL && pair_live;
+    int x[4];
+    coords(L, pair_live ? xs : 0, x);
+    const long xr = fwd(L, pair_live ? xs : 0, x, k + 1, R);  // spatial index of x + R k
+    const long kV = (long)9 * k * V;
+
+    // one time slice of the block (sinks S_j and the links of usrc at x and
+    // x + R k) into registers / from registers into a ring slot
+    v2<TS> ps[PS];
+    v2<TC> pu[PU];
+    auto fetch_at = [&](long ys, const v2<TC>* __restrict__ usrc) {
+#pragma unroll
+      for (int m = 0; m < PS; m++) {
+        ps[m].x = TS(0);
+        ps[m].y = TS(0);
+        if (sj_[m] >= 0 && xs0 + sp_[m] < slice) ps[m] = S.p[sj_[m]][kV + so_[m] + ys + xs0 + sp_[m]];
+      }
+#pragma unroll
+      for (int m = 0; m < PU; m++) {
+        pu[m].x = TC(0);
+        pu[m].y = TC(0);
+        if (uh_[m] >= 0 && xs0 + up_[m] < slice)
+          pu[m] = usrc[uo_[m] + ys + (uh_[m] ? xr_tab[up_[m]] : xs0 + up_[m])];
+      }
+    };
+    // ring offset o: time slice x0g+Ta+o
+    auto fetch = [&](int o) { fetch_at((long)((x0g + Ta + o) % N0) * slice, U0); };
+    auto put = [&](int slot) {
+      char* dst = ring + slot * SLOT;
+#pragma unroll
+      for (int m = 0; m < PS; m++)
+        if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT] = ps[m];
+#pragma unroll
+      for (int m = 0; m < PU; m++)
+        if (uh_[m] >= 0) ((v2<TC>*)(dst + NS * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = pu[m];
+    };
+    // (up to three slices in flight, as the register budget of the occupancy
+    // target allows)
+    constexpr int FILL = (sizeof(TC) == 8 && W < 8) ? 3 : 5;
+
+    if (threadIdx.x < 16) xr_tab[threadIdx.x] = xr;
+    __syncthreads();  // xr_tab; the previous block's last reads of the ring
+    // the start slices x0g+w with S_i(x) and L_Ta at x and x + R k: for Ta = 1
+    // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
+    // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
+    if (Ta == 1) {
+#pragma unroll FILL
+      for (int o = -1; o < W; o++) {
+        fetch(o);
+        put((o + W + 1) % (W + 1));
+      }
+    } else {
+#pragma unroll FILL
+      for (int o = 0; o < W; o++) {
+        fetch_at((long)((x0g + o) % N0) * slice, LTa);
+        put(o);
+      }
+    }
+    __syncthreads();
+    // D_i(Ta) = L_Ta(x)^dag S_i(x) L_Ta(x + R k)
+    M3<TC> D;
+    set_zero(D);
+    if (live) {
+      const char* slot = ring + (Ta == 1 ? (w + W) % (W + 1) : w) * SLOT;
+      const v2<TS>* rs = (const v2<TS>*)slot + p;
+      const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
+      M3<TC> a;
+      M3<TS> t;
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        v2<TS> v = rs[(i * 9 + e) * 16];
+        t.re[e] = v.x;
+        t.im[e] = v.y;
+        v2<TC> c = ru[e * 16];
+        a.re[e] = c.x;
+        a.im[e] = c.y;
+      }
+      const M3<TC> h = dag_mul(a, convert<TC>(t));
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        v2<TC> c = ru[(9 + e) * 16];
+        a.re[e] = c.x;
+        a.im[e] = c.y;
+      }
+      D = mul(h, a);
+    }
+    __syncthreads();  // ring slots of the start slices are refilled next
+    if (Ta != 1) {
+#pragma unroll FILL
+      for (int o = 0; o < W; o++) {
+        fetch(o);
+        put(o);
+      }
+      __syncthreads();
+    }
+
+    for (int n = 0; n < d; n++) {
+      const bool more = n + 1 < d;
+      if (more) fetch(n + W);  // the slice wave W-1 needs at step n+1
+      const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
+      const v2<TS>* rs = (const v2<TS>*)slot + p;
+      const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
+      // traces against the sinks S_j(x + (Ta+n) 0)
+      double tr[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        tr[j] = 0.0;
+        if (j < MSL) {
+          M3<TS> sj;
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            v2<TS> v = rs[(j * 9 + e) * 16];
+            sj.re[e] = v.x;
+            sj.im[e] = v.y;
+          }
+          tr[j] = (double)retr_dag_mul(convert<TC>(sj), D);
+        }
+      }
+      // sum over the 16 pairs of the row; lane p ends with the sum for j = p >> 2
+      const bool b3 = (p & 8) != 0, b2 = (p & 4) != 0;
+      double s0 = b3 ? tr[0] : tr[2], s1 = b3 ? tr[1] : tr[3];
+      double k0 = b3 ? tr[2] : tr[0], k1 = b3 ? tr[3] : tr[1];
+      k0 += __shfl_xor(s0, 8, 64);
+      k1 += __shfl_xor(s1, 8, 64);
+      const double sh = b2 ? k0 : k1;
+      double v = b2 ? k1 : k0;
+      v += __shfl_xor(sh, 4, 64);
+      v += __shfl_xor(v, 2, 64);
+      v += __shfl_xor(v, 1, 64);
+      if (wave_live && i < MSL && (p & 3) == 0 && (p >> 2) < MSL)
+        acc[(w * 16 + n) * 16 + i * MSL + (p >> 2)] += v;
+      if (more) {
+        if (live) {
+          M3<TC> u;
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            v2<TC> c = ru[e * 16];
+            u.re[e] = c.x;
+            u.im[e] = c.y;
+          }
+          M3<TC> h = dag_mul(u, D);
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            v2<TC> c = ru[(9 + e) * 16];
+            u.re[e] = c.x;
+            u.im[e] = c.y;
+          }
+          D = mul(h, u);
+        }
+        put((n + W) % (W + 1));
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < W * d * MM; q += NT) {
+    const int ww = q / (d * MM), n = (q / MM) % d, ij = q % MM;
+    const int xo = x0g + ww;
+    if (xo < N0)
+      partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] = acc[(ww * 16 + n) * 16 + ij];
+  }
+}
+
 // W_ij(R,T;x0) from the partials of every (R,T) launch, in wl_idx order.
 // partial layout per (R,T): [x0][slot][i*msl+j] (nwg slots per x0); the slot
 // sums are added in fixed order with Neumaier compensation, then normalised.
@@ -447,6 +694,45 @@ static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
                      sp, partial);
 }
 
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes) {
+  const size_t slot = (size_t)msl * 9 * 16 * 2 * ts_bytes + (size_t)2 * 9 * 16 * 2 * tc_bytes;
+  return (size_t)W * 256 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot;
+}
+
+template <int MSL, int W, class TS, class TC>
+static void depth_launch(const Lat& L, int R, int Ta, int d, int nslot, const v2<TC>* LTa,
+                         const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
+                         hipStream_t st) {
+  const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC));
+  const dim3 grid((unsigned)(((L.n[0] + W - 1) / W) * nslot));
+  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC>), grid, dim3(64 * W), lds, st, L, R, Ta, d,
+                     nslot, LTa, U0, sp, partial, pstride);
+}
+
+template <int MSL, class TS, class TC>
+static void depth_launch_w(const Lat& L, int R, int Ta, int d, int W, int nslot, const v2<TC>* LTa,
+                           const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
+                           hipStream_t st) {
+  if (W >= 8)
+    depth_launch<MSL, 8, TS, TC>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+  else
+    depth_launch<MSL, 4, TS, TC>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+}
+
+template <class TS, class TC>
+void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
+                           const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
+                           double* partial, long pstride, hipStream_t st) {
+  LevelPtrs<TS> sp;
+  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
+  switch (msl) {
+    case 1: depth_launch_w<1, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 2: depth_launch_w<2, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 3: depth_launch_w<3, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    default: depth_launch_w<4, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+  }
+}
+
 // kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl
 // lanes per pair) with wg persistent workgroups; 2: one-lane chain form with
 // wg persistent workgroups (msl <= 4; otherwise the streaming form).
@@ -510,5 +796,16 @@ template void launch_contract<double, double>(const Lat&, int, int, int, const d
                                               hipStream_t);
 template void launch_contract<float, float>(const Lat&, int, int, int, const float2*,
                                             const float2* const*, double*, int, int, hipStream_t);
+template void launch_contract_depth<double, double>(const Lat&, int, int, int, int, int, int,
+                                                    const double2*, const double2*,
+                                                    const double2* const*, double*, long,
+                                                    hipStream_t);
+template void launch_contract_depth<float, float>(const Lat&, int, int, int, int, int, int,
+                                                  const float2*, const float2*,
+                                                  const float2* const*, double*, long, hipStream_t);
+template void launch_contract_depth<float, double>(const Lat&, int, int, int, int, int, int,
+                                                   const double2*, const double2*,
+                                                   const float2* const*, double*, long,
+                                                   hipStream_t);
 
 }  // namespace wl
