@@ -357,6 +357,17 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
   }
 }
 
+// ---- lane exchange within a row of 16 lanes (DPP) ------------------------
+// partner of lane p: 15 - p (mirror), 7 - p within each half of 8 (half
+// mirror), p ^ 2 and p ^ 1 (quad permutations)
+constexpr int kRowMirror = 0x140, kRowHalfMirror = 0x141, kQuadXor2 = 0x4E, kQuadXor1 = 0xB1;
+
+template <int CTRL> __device__ __forceinline__ double row_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 // ---- contraction, fused time-slice form ----------------------------------
 // One launch covers T = Ta .. Ta+d-1 of one R.  A lane owns (pair p of a block
 // of 16 (site,k) pairs, level i) for one start slice x0 and carries
@@ -376,9 +387,10 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
 // once, and no global load is waited for inside a step.
 //
 // Lane layout: lane = 16 i + p (a row of 16 lanes per level).  The traces of a
-// step are summed over the 16 pairs of a row in FP64 by a butterfly that
-// hands half of the carried values to the partner in each of its first two
-// stages, then added into a per-wave LDS accumulator acc[T][i][j]; one
+// step are summed over the 16 pairs of a row in FP64 by a butterfly of DPP
+// exchanges (partners 15-p, the half mirror, p^2, p^1) that hands half of the
+// carried values to the partner in each of its first two stages, then added
+// into a per-wave LDS accumulator acc[T][i][j]; one
 // partial per (T, x0, workgroup slot, i, j) is stored at the end.  Workgroup
 // b serves start group b / nslot and pair blocks b % nslot + m nslot, so the
 // workgroups that read the same pair blocks sit on the same XCD when nslot is
@@ -563,13 +575,13 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       const bool b3 = (p & 8) != 0, b2 = (p & 4) != 0;
       double s0 = b3 ? tr[0] : tr[2], s1 = b3 ? tr[1] : tr[3];
       double k0 = b3 ? tr[2] : tr[0], k1 = b3 ? tr[3] : tr[1];
-      k0 += __shfl_xor(s0, 8, 64);
-      k1 += __shfl_xor(s1, 8, 64);
+      k0 += row_dpp<kRowMirror>(s0);
+      k1 += row_dpp<kRowMirror>(s1);
       const double sh = b2 ? k0 : k1;
       double v = b2 ? k1 : k0;
-      v += __shfl_xor(sh, 4, 64);
-      v += __shfl_xor(v, 2, 64);
-      v += __shfl_xor(v, 1, 64);
+      v += row_dpp<kRowHalfMirror>(sh);
+      v += row_dpp<kQuadXor2>(v);
+      v += row_dpp<kQuadXor1>(v);
       if (wave_live && i < MSL && (p & 3) == 0 && (p >> 2) < MSL)
         acc[(w * 16 + n) * 16 + i * MSL + (p >> 2)] += v;
       if (more) {
