@@ -130,9 +130,10 @@ extern "C" int wlg_create(wlg_ctx** out, const int dims[4], int bc, int device) 
   c->L.st[1] = (long)dims[2] * dims[3];
   c->L.st[0] = (long)dims[1] * dims[2] * dims[3];
   c->L.V = (long)dims[0] * c->L.st[0];
-  if (c->L.V > 0x7fffffffL) {
+  if (c->L.V >= (1L << 28)) {  // 32-bit byte offsets of a site within a field plane
+    const long V = c->L.V;
     delete c;
-    return set_err("lattice volume %ld exceeds 2^31-1 sites", c->L.V);
+    return set_err("lattice volume %ld exceeds 2^28-1 sites", V);
   }
   hipError_t e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc((void**)&c->U, 4 * slot64(c->L));

@@ -17,34 +17,32 @@ namespace wl {
 // v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
 // ln: slot of the nu-links in A, lm: slot of the mu-links in B.
 template <class TC, class TS>
-__device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, long s, const int x[4],
+__device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, site_t s, const int x[4],
                                            const v2<TS>* __restrict__ A, int ln,
                                            const v2<TS>* __restrict__ B, int lm, int mu, int nu) {
   M3<TS> a, b, c;
-  long xpn = fwd(L, s, x, nu, 1);
-  long xpm = fwd(L, s, x, mu, 1);
+  const site_t xpn = fwd(L, s, x, nu, 1);
+  const site_t xpm = fwd(L, s, x, mu, 1);
   load(b, B, L.V, lm, xpn);
   load(c, A, L.V, ln, xpm);
   load(a, A, L.V, ln, s);
   M3<TC> t = mul_dag(convert<TC>(b), convert<TC>(c));
-  t = mul(convert<TC>(a), t);
-  acc(v, t);
-  long xmn = bwd(L, s, x, nu);
+  mul_acc(v, convert<TC>(a), t);
+  const site_t xmn = bwd(L, s, x, nu);
   int y[4] = {x[0], x[1], x[2], x[3]};
   y[nu] = (x[nu] == 0) ? L.n[nu] - 1 : x[nu] - 1;
-  long xmnpm = fwd(L, xmn, y, mu, 1);
+  const site_t xmnpm = fwd(L, xmn, y, mu, 1);
   load(b, B, L.V, lm, xmn);
   load(c, A, L.V, ln, xmnpm);
   load(a, A, L.V, ln, xmn);
   t = mul(convert<TC>(b), convert<TC>(c));
-  t = dag_mul(convert<TC>(a), t);
-  acc(v, t);
+  dag_mul_acc(v, convert<TC>(a), t);
 }
 
 // out = P((1-a)U + b S), S in TC, U and out in TS, combination in FP64
 template <class TC, class TS>
 __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __restrict__ U, int lu,
-                                               v2<TS>* __restrict__ out, int lo, long V, long s,
+                                               v2<TS>* __restrict__ out, int lo, long V, site_t s,
                                                double alpha, double beta, int proj, int piter) {
   M3<TS> u;
   load(u, U, V, lu, s);
@@ -71,7 +69,7 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restrict__ U,
                                                   v2<TS>* __restrict__ D1, double alpha,
                                                   double beta, int proj, int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = blockIdx.y / 3, eta = (mu + 1 + blockIdx.y % 3) & 3;
   int x[4];
@@ -89,7 +87,7 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
                                                   const v2<TS>* __restrict__ D1,
                                                   v2<TS>* __restrict__ D2, double alpha,
                                                   double beta, int proj, int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = blockIdx.y / 3, nu = (mu + 1 + blockIdx.y % 3) & 3;
   int n = part4(mu, nu);
@@ -112,7 +110,7 @@ __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>
                                                   const v2<TS>* __restrict__ D2, int mu0,
                                                   double alpha, double beta, int proj,
                                                   int piter, int open_bc) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = mu0 + blockIdx.y;
   int x[4];
@@ -143,7 +141,7 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256) void hyp3_level1(Lat L, const v2<TS>* __restrict__ F,
                                                    v2<TS>* __restrict__ D, double alpha,
                                                    double beta, int proj, int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = 1 + blockIdx.y / 2, eta = ((mu + blockIdx.y % 2) % 3) + 1;
   int x[4];
@@ -162,7 +160,7 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
                                                    const v2<TS>* __restrict__ D,
                                                    v2<TS>* __restrict__ out, double alpha,
                                                    double beta, int proj, int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = 1 + blockIdx.y;
   int x[4];
@@ -180,12 +178,13 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
 // One APE step on spatial links (ape_links.c:147-262 with time = 0):
 // out_mu = P(in_mu + alpha * sum_{eta != mu, spatial, ascending} S_{mu eta}[in,in]).
 // in and out are 3-slot spatial fields (slot k = mu-1); Jacobi update.
-// FP32 links: 4 waves per SIMD fit without spills (122 VGPRs)
+// register targets (waves per SIMD without spills): 4 with FP32 links, 3 with
+// FP64 links and FP32 staples, 2 in FP64
 template <class TC, class TS>
-__global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : 1) void ape_step(Lat L, const v2<TS>* __restrict__ in,
+__global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2)) void ape_step(Lat L, const v2<TS>* __restrict__ in,
                                                 v2<TS>* __restrict__ out, double alpha, int proj,
                                                 int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = 1 + blockIdx.y;
   int x[4];
@@ -204,7 +203,7 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict__ in,
                                                  v2<TS>* __restrict__ out, double alpha, int proj,
                                                  int piter) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int mu = blockIdx.y;
   int x[4];
@@ -222,7 +221,7 @@ __global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict
 __global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* __restrict__ A,
                                                          const double2* __restrict__ B,
                                                          double2* __restrict__ v, int mu, int nu) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int x[4];
   coords(L, s, x);

@@ -35,27 +35,37 @@ template <class D, class S> __device__ __forceinline__ M3<D> convert(const M3<S>
 }
 
 // ---- HBM access ----------------------------------------------------------
+// Site indices are 32-bit and so are the byte offsets of a site within a plane
+// (the volume is below 2^28, checked when a context is created): the address of
+// element e of slot l is the plane base f + (9 l + e) V, uniform across a
+// wavefront when l is, plus a 32-bit per-lane byte offset.
+using site_t = uint32_t;
+
+template <class T> __device__ __forceinline__ const T* at(const T* plane, site_t s) {
+  return (const T*)((const char*)plane + (uint32_t)(s * (uint32_t)sizeof(T)));
+}
+template <class T> __device__ __forceinline__ T* at(T* plane, site_t s) {
+  return (T*)((char*)plane + (uint32_t)(s * (uint32_t)sizeof(T)));
+}
 
 template <class R>
-__device__ __forceinline__ void load(M3<R>& m, const v2<R>* __restrict__ f, long V, int l, long s) {
-  const v2<R>* p = f + (long)l * 9 * V + s;
+__device__ __forceinline__ void load(M3<R>& m, const v2<R>* __restrict__ f, long V, int l, site_t s) {
 #pragma unroll
   for (int e = 0; e < 9; e++) {
-    v2<R> v = p[(long)e * V];
+    v2<R> v = *at(f + (long)(9 * l + e) * V, s);
     m.re[e] = v.x;
     m.im[e] = v.y;
   }
 }
 
 template <class R>
-__device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, long s) {
-  v2<R>* p = f + (long)l * 9 * V + s;
+__device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, site_t s) {
 #pragma unroll
   for (int e = 0; e < 9; e++) {
     v2<R> v;
     v.x = m.re[e];
     v.y = m.im[e];
-    p[(long)e * V] = v;
+    *at(f + (long)(9 * l + e) * V, s) = v;
   }
 }
 
@@ -141,6 +151,45 @@ template <class R> __device__ __forceinline__ M3<R> dag_mul(const M3<R>& a, cons
   return w;
 }
 
+// v += a * b
+template <class R> __device__ __forceinline__ void mul_acc(M3<R>& v, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = v.re[3 * i + k], im = v.im[3 * i + k];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        re = fmadd(a.re[3 * i + j], b.re[3 * j + k], re);
+        im = fmadd(a.re[3 * i + j], b.im[3 * j + k], im);
+        re = fmadd(-a.im[3 * i + j], b.im[3 * j + k], re);
+        im = fmadd(a.im[3 * i + j], b.re[3 * j + k], im);
+      }
+      v.re[3 * i + k] = re;
+      v.im[3 * i + k] = im;
+    }
+}
+
+// v += a^dag * b
+template <class R>
+__device__ __forceinline__ void dag_mul_acc(M3<R>& v, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = v.re[3 * i + k], im = v.im[3 * i + k];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        re = fmadd(a.re[3 * j + i], b.re[3 * j + k], re);
+        im = fmadd(a.re[3 * j + i], b.im[3 * j + k], im);
+        re = fmadd(a.im[3 * j + i], b.im[3 * j + k], re);
+        im = fmadd(-a.im[3 * j + i], b.re[3 * j + k], im);
+      }
+      v.re[3 * i + k] = re;
+      v.im[3 * i + k] = im;
+    }
+}
+
 template <class R> __device__ __forceinline__ void acc(M3<R>& v, const M3<R>& a) {
 #pragma unroll
   for (int e = 0; e < 9; e++) { v.re[e] += a.re[e]; v.im[e] += a.im[e]; }
@@ -171,40 +220,71 @@ template <class R> __device__ __forceinline__ R retr_dag_mul(const M3<R>& a, con
 __device__ __forceinline__ double det_im(const M3<double>& u) {
   const double* r = u.re;
   const double* i = u.im;
-  double d1r = (r[4] * r[8] - i[4] * i[8]) - (r[5] * r[7] - i[5] * i[7]);
-  double d1i = (r[4] * i[8] + i[4] * r[8]) - (r[5] * i[7] + i[5] * r[7]);
-  double d2r = (r[3] * r[8] - i[3] * i[8]) - (r[5] * r[6] - i[5] * i[6]);
-  double d2i = (r[3] * i[8] + i[3] * r[8]) - (r[5] * i[6] + i[5] * r[6]);
-  double d3r = (r[3] * r[7] - i[3] * i[7]) - (r[4] * r[6] - i[4] * i[6]);
-  double d3i = (r[3] * i[7] + i[3] * r[7]) - (r[4] * i[6] + i[4] * r[6]);
-  return (r[0] * d1i + i[0] * d1r) - (r[1] * d2i + i[1] * d2r) + (r[2] * d3i + i[2] * d3r);
+  // minors d1 = u4 u8 - u5 u7, d2 = u3 u8 - u5 u6, d3 = u3 u7 - u4 u6
+  double d1r = r[4] * r[8], d1i = r[4] * i[8];
+  d1r = fmadd(-i[4], i[8], d1r);
+  d1i = fmadd(i[4], r[8], d1i);
+  d1r = fmadd(-r[5], r[7], d1r);
+  d1i = fmadd(-r[5], i[7], d1i);
+  d1r = fmadd(i[5], i[7], d1r);
+  d1i = fmadd(-i[5], r[7], d1i);
+  double d2r = r[3] * r[8], d2i = r[3] * i[8];
+  d2r = fmadd(-i[3], i[8], d2r);
+  d2i = fmadd(i[3], r[8], d2i);
+  d2r = fmadd(-r[5], r[6], d2r);
+  d2i = fmadd(-r[5], i[6], d2i);
+  d2r = fmadd(i[5], i[6], d2r);
+  d2i = fmadd(-i[5], r[6], d2i);
+  double d3r = r[3] * r[7], d3i = r[3] * i[7];
+  d3r = fmadd(-i[3], i[7], d3r);
+  d3i = fmadd(i[3], r[7], d3i);
+  d3r = fmadd(-r[4], r[6], d3r);
+  d3i = fmadd(-r[4], i[6], d3i);
+  d3r = fmadd(i[4], i[6], d3r);
+  d3i = fmadd(-i[4], r[6], d3i);
+  // Im (u0 d1 - u1 d2 + u2 d3)
+  double t = r[0] * d1i;
+  t = fmadd(i[0], d1r, t);
+  t = fmadd(-r[1], d2i, t);
+  t = fmadd(-i[1], d2r, t);
+  t = fmadd(r[2], d3i, t);
+  return fmadd(i[2], d3r, t);
 }
 
 __device__ __forceinline__ void project(M3<double>& u, int iter) {
   double d = 0.0;
 #pragma unroll
-  for (int e = 0; e < 9; e++) d += u.re[e] * u.re[e] + u.im[e] * u.im[e];
+  for (int e = 0; e < 9; e++) {
+    d = fmadd(u.re[e], u.re[e], d);
+    d = fmadd(u.im[e], u.im[e], d);
+  }
   if (d < 4.242640687119285 * 2.220446049250313e-16) return;
   d = 1.0 / sqrt(d / 3.0);
 #pragma unroll
   for (int e = 0; e < 9; e++) { u.re[e] *= d; u.im[e] *= d; }
   for (int it = 0; it < iter; it++) {
-    // h = u^dag u is Hermitian: build the 6 independent entries
+    // x = (3 - u^dag u)/2, Hermitian: the 6 independent entries of u^dag u
     M3<double> x;
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int b = a; b < 3; b++) {
-        double re = u.re[a] * u.re[b] + u.im[a] * u.im[b];
-        double im = u.re[a] * u.im[b] - u.im[a] * u.re[b];
-        re += u.re[3 + a] * u.re[3 + b] + u.im[3 + a] * u.im[3 + b];
-        im += u.re[3 + a] * u.im[3 + b] - u.im[3 + a] * u.re[3 + b];
-        re += u.re[6 + a] * u.re[6 + b] + u.im[6 + a] * u.im[6 + b];
-        im += u.re[6 + a] * u.im[6 + b] - u.im[6 + a] * u.re[6 + b];
+        double re = u.re[a] * u.re[b];
+        re = fmadd(u.im[a], u.im[b], re);
+        re = fmadd(u.re[3 + a], u.re[3 + b], re);
+        re = fmadd(u.im[3 + a], u.im[3 + b], re);
+        re = fmadd(u.re[6 + a], u.re[6 + b], re);
+        re = fmadd(u.im[6 + a], u.im[6 + b], re);
         if (a == b) {
-          x.re[3 * a + a] = 1.5 - 0.5 * re;
+          x.re[3 * a + a] = fmadd(-0.5, re, 1.5);
           x.im[3 * a + a] = 0.0;
         } else {
+          double im = u.re[a] * u.im[b];
+          im = fmadd(-u.im[a], u.re[b], im);
+          im = fmadd(u.re[3 + a], u.im[3 + b], im);
+          im = fmadd(-u.im[3 + a], u.re[3 + b], im);
+          im = fmadd(u.re[6 + a], u.im[6 + b], im);
+          im = fmadd(-u.im[6 + a], u.re[6 + b], im);
           x.re[3 * a + b] = -0.5 * re;
           x.im[3 * a + b] = -0.5 * im;
           x.re[3 * b + a] = -0.5 * re;
@@ -212,11 +292,11 @@ __device__ __forceinline__ void project(M3<double>& u, int iter) {
         }
       }
     x = mul(u, x);
-    double zi = det_im(x) * (-1.0 / 3.0);
+    const double zi = det_im(x) * (-1.0 / 3.0);
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      u.re[e] = x.re[e] - zi * x.im[e];
-      u.im[e] = x.im[e] + zi * x.re[e];
+      u.re[e] = fmadd(-zi, x.im[e], x.re[e]);
+      u.im[e] = fmadd(zi, x.re[e], x.im[e]);
     }
   }
 }
@@ -230,22 +310,22 @@ struct Lat {
   long V;
 };
 
-__device__ __forceinline__ void coords(const Lat& L, long s, int x[4]) {
-  long r = s;
-  x[3] = (int)(r % L.n[3]); r /= L.n[3];
-  x[2] = (int)(r % L.n[2]); r /= L.n[2];
-  x[1] = (int)(r % L.n[1]); r /= L.n[1];
+__device__ __forceinline__ void coords(const Lat& L, site_t s, int x[4]) {
+  site_t r = s;
+  x[3] = (int)(r % (site_t)L.n[3]); r /= (site_t)L.n[3];
+  x[2] = (int)(r % (site_t)L.n[2]); r /= (site_t)L.n[2];
+  x[1] = (int)(r % (site_t)L.n[1]); r /= (site_t)L.n[1];
   x[0] = (int)r;
 }
 
-// site reached from s (coordinates x) by d >= 0 steps in direction mu
-__device__ __forceinline__ long fwd(const Lat& L, long s, const int x[4], int mu, int d) {
+// site reached from s (coordinates x) by 0 <= d < N_mu steps in direction mu
+__device__ __forceinline__ site_t fwd(const Lat& L, site_t s, const int x[4], int mu, int d) {
   int y = x[mu] + d;
-  return (y >= L.n[mu]) ? s + (long)(d - L.n[mu]) * L.st[mu] : s + (long)d * L.st[mu];
+  return (y >= L.n[mu]) ? s + (site_t)((long)(d - L.n[mu]) * L.st[mu]) : s + (site_t)((long)d * L.st[mu]);
 }
 
-__device__ __forceinline__ long bwd(const Lat& L, long s, const int x[4], int mu) {
-  return (x[mu] == 0) ? s + (long)(L.n[mu] - 1) * L.st[mu] : s - L.st[mu];
+__device__ __forceinline__ site_t bwd(const Lat& L, site_t s, const int x[4], int mu) {
+  return (x[mu] == 0) ? s + (site_t)((long)(L.n[mu] - 1) * L.st[mu]) : s - (site_t)L.st[mu];
 }
 
 }  // namespace wl

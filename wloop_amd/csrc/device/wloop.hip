@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void time_transporter(Lat L, const v2<TU>* __r
                                                         int lu, const v2<TP>* __restrict__ prev,
                                                         v2<TO>* __restrict__ out,
                                                         v2<float>* __restrict__ out32) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int x[4];
   coords(L, s, x);
@@ -51,7 +51,7 @@ template <class TS>
 __global__ __launch_bounds__(256) void space_transporter(Lat L, LevelPtrs<TS> F,
                                                          LevelPtrs<TS> prev,
                                                          v2<TS>* __restrict__ out) {
-  long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= L.V) return;
   int i = blockIdx.y / 3, k = blockIdx.y % 3;
   int x[4];
