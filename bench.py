@@ -208,12 +208,14 @@ def main():
     w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
     log(f"[bench] fp64 (plain): {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
     best64 = None
-    if not args.no_autotune and choice["fp64"]["kernel"] == W.KERNEL_DEPTH:
-        autotune.apply(lat, choice["fp64"], W.FP64)
+    if args.no_autotune or choice["fp64"]["kernel"] == W.KERNEL_DEPTH:
+        # the fused time-slice form for FP64 (library default shape without autotuning)
+        autotune.apply(lat, {"kernel": W.KERNEL_DEPTH, "wg": 0} if args.no_autotune else choice["fp64"],
+                       W.FP64)
         w64b, t64b, lat64b, tim64b, cs64b = time_path(lat, parms, W.FP64, args.steps, args.warmup)
         log(f"[bench] fp64 (fused time slices): {t64b / args.steps * 1e3:.1f} ms/config  stages {tim64b}")
         best64 = {"value": args.steps / t64b, "ms_per_step": t64b / args.steps * 1e3,
-                  "contraction": tuned["fp64"],
+                  "contraction": tuned["fp64"] if choice else "fused time slices, library default shape",
                   "latency_ms": {"p50": float(np.percentile(lat64b, 50)),
                                  "p99": float(np.percentile(lat64b, 99))},
                   "stage_ms_per_config": {k: v / args.steps for k, v in tim64b.items()},

@@ -398,9 +398,10 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   // form for the FP32 transporters of the mixed path
   k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 0);
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  k.wg = tw > 0 ? tw : (k.kernel == 3 ? 32 : k.kernel >= 1 ? 512 : 8);
+  static const int wg_default[4] = {8, 512, 1024, 128};  // per form, measured at 48^3x96
+  k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
-  k.starts = ts > 0 ? ts : 4;
+  k.starts = ts > 0 ? ts : 8;
   // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
   // time links on the FP64 blocks of the mixed path)
   const int tsb = mixed ? 4 : 8;
