@@ -74,9 +74,10 @@ def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=Fals
     if not force and key in db:
         apply(lat, db[key], precision)
         return dict(db[key], cached=True)
-    # a short measurement with the real level count and T range: no APE steps,
-    # two values of R
-    trial = W.make_parms([0] * parms.msl, parms.mwlt, min(parms.mwlr, 2), proj_iter=parms.proj_iter)
+    # a short measurement with the real level count and T range: distinct
+    # levels one APE step apart, two values of R
+    trial = W.make_parms([0] + [1] * (parms.msl - 1), parms.mwlt, min(parms.mwlr, 2),
+                         proj_iter=parms.proj_iter)
     cands = candidates or CANDIDATES
     trials = []
     for cand in cands:

@@ -241,6 +241,14 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round robin (workgroup
+// b to XCD b % 8), each XCD with its own L2.  xcd_order maps b to a position
+// such that every XCD works through one contiguous range of positions.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int q = n / 8, r = n % 8, x = b % 8, l = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
 // ---- contraction, one-lane chain form ----------------------------------
 // As the chain form, but one lane owns a (site, k) pair with the open loops
 // D_i of all msl levels, so every sink link is loaded by one lane only.  Per
@@ -248,7 +256,11 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
 // transposed through LDS and summed in FP64 (16 lanes per value, then two
 // butterfly steps); the 4 wave sums are added in fixed order into a per-
 // workgroup FP64 accumulator indexed by x0 (one barrier per step: all waves
-// of a workgroup walk the same chain).  Requires msl <= 4.
+// of a workgroup walk the same chain).  Work units are blocks of 256 sites of
+// one direction k, the three directions of a block next to each other, and
+// each XCD takes a contiguous range of units: the time transporters L_T(x) and
+// L_T(x + R k) of a block are fetched into that XCD's L2 once for the three
+// directions.  Requires msl <= 4.
 template <int MSL, class TS, class TC>
 __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int g,
                                                        const v2<TS>* __restrict__ LT,
@@ -263,8 +275,8 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
   TC* redf = (TC*)(red2 + 2 * 4 * 16);                    // [4][64][16], traces in TC
   const int len = N0 / g;
   const long slice = L.st[0];
-  const long npairs = 3 * slice;
-  const long nblocks = (npairs + 255) / 256;
+  const long nbk = (slice + 255) / 256;  // blocks of 256 sites per direction
+  const long nblocks = 3 * nbk;
   const long units = nblocks * g;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   TC* rw = redf + wave * 64 * 16;
@@ -272,12 +284,13 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
   for (int q = threadIdx.x; q < N0 * MM; q += 256) acc[q] = 0.0;
   __syncthreads();
   int par = 0;
-  for (long u = blockIdx.x; u < units; u += gridDim.x) {
+  for (long u = xcd_order(blockIdx.x, gridDim.x); u < units; u += gridDim.x) {
     const int c = (int)(u / nblocks);
-    const long pr = (u % nblocks) * 256 + threadIdx.x;
-    const bool live = pr < npairs;
-    const int k = live ? (int)(pr / slice) : 0;
-    const long xs = live ? pr % slice : 0;
+    const long bl = u % nblocks;
+    const int k = (int)(bl % 3);
+    const long xs0 = (bl / 3) * 256 + threadIdx.x;
+    const bool live = xs0 < slice;
+    const long xs = live ? xs0 : 0;
     int x[4];
     coords(L, xs, x);
     M3<TC> D[MSL], Sx[MSL];
