@@ -398,7 +398,7 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   // form for the FP32 transporters of the mixed path
   k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 0);
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[4] = {8, 512, 1024, 128};  // per form, measured at 48^3x96
+  static const int wg_default[4] = {8, 512, 512, 128};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : 8;
@@ -409,8 +409,10 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   return k;
 }
 
+// partial sums of the (R,T) launches of one R (W of an R is finished before
+// the next R is contracted)
 static size_t partial_count(const Lat& L, const wlg_parms* p, const Contract& k) {
-  return (size_t)p->mwlr * p->mwlt * L.n[0] * contract_slots(L, k.kernel, k.wg) * p->msl * p->msl;
+  return (size_t)p->mwlt * L.n[0] * contract_slots(L, k.kernel, k.wg) * p->msl * p->msl;
 }
 
 extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
@@ -421,13 +423,12 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   L.st[1] = (long)dims[2] * dims[3];
   L.st[0] = (long)dims[1] * dims[2] * dims[3];
   L.V = (long)dims[0] * L.st[0];
-  // default launch shapes: FP64 time transporters of depth-16 blocks, and
-  // the partial sums of the streaming form with 32 chunks (the largest)
+  // default launch shapes: FP64 time transporters of depth-16 blocks
   Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16));
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  const Contract k = {0, 32, 16, 4};
+  const Contract k = {0, 32, 16, 8};  // the most partial sums among the tuning candidates
   return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, k) * sizeof(double);
 }
 
@@ -499,7 +500,7 @@ template <class TS, class TC>
 static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* const* lev,
                                   const v2<TS>* const* LT, v2<TS>* sbuf0, v2<TS>* sbuf1,
                                   const Contract& k, const std::vector<DepthBlock>& blocks,
-                                  const std::vector<DepthPtrs>& bptr) {
+                                  const std::vector<DepthPtrs>& bptr, double norm) {
   const Lat& L = c->L;
   const int msl = p->msl;
   const size_t lvl = (size_t)27 * L.V;  // one 3-slot field in 2-vectors
@@ -524,7 +525,7 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     if (k.kernel == 3) {
       for (size_t q = 0; q < blocks.size(); q++) {
         const DepthBlock& b = blocks[q];
-        double* part = c->partial + (size_t)(r * p->mwlt + b.Ta - 1) * pstride;
+        double* part = c->partial + (size_t)(b.Ta - 1) * pstride;
         if constexpr (sizeof(TS) == 4) {
           if (b.fp64)
             launch_contract_depth<float, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
@@ -547,13 +548,16 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
       c->contract_launches += (long)blocks.size();
     } else {
       for (int t = 0; t < p->mwlt; t++) {
-        launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S,
-                                c->partial + (size_t)(r * p->mwlt + t) * pstride, k.kernel, k.wg,
-                                c->st);
+        launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride,
+                                k.kernel, k.wg, c->st);
         CKL();
       }
       c->contract_launches += p->mwlt;
     }
+    // W of this R (normalisation and wl_idx store, wloop.c:535, :584-597)
+    launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, r, contract_slots(L, k.kernel, k.wg),
+                        c->partial, norm, c->wl_dev, c->st);
+    CKL();
     if (record(c, &e2)) return -1;
   }
   return 0;
@@ -574,7 +578,6 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  const int nslot = contract_slots(L, k.kernel, k.wg);
   if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
     return -1;
   const size_t s64 = slot64(L), s32 = slot32(L);
@@ -585,6 +588,8 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   if (run_levels(c, U, p, prec, o, lev, lev32p)) return -1;
   hipEvent_t e_lev;
   if (record(c, &e_lev)) return -1;
+
+  const double norm = 9.0 * L.n[1] * L.n[2] * L.n[3];  // wloop.c:535
 
   // ---- time transporters L_T (wloop.c:153-204) ----
   // per-(R,T) forms: every T; fused form: the starts of its T blocks only
@@ -636,7 +641,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     ev_before_rt = c->ev_used;
     float2* sb = (float2*)(A + o.s);
     if (run_transport_contract<float, float>(c, p, lev32p, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                             k, blocks, bptr))
+                                             k, blocks, bptr, norm))
       return -1;
     (void)s32;
   } else {
@@ -655,15 +660,11 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     ev_before_rt = c->ev_used;
     double2* sb = (double2*)(A + o.s);
     if (run_transport_contract<double, double>(c, p, lev, lts.data(), sb, sb + 3 * msl * 9 * L.V,
-                                               k, blocks, bptr))
+                                               k, blocks, bptr, norm))
       return -1;
   }
   size_t ev_after_rt = c->ev_used;
 
-  // ---- normalisation and wl_idx store (wloop.c:535, :584-597) ----
-  double norm = 9.0 * L.n[1] * L.n[2] * L.n[3];
-  launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, nslot, c->partial, norm, c->wl_dev, c->st);
-  CKL();
   if (c->graph) {
     CK(hipMemcpyAsync(c->wl_pinned, c->wl_dev, wls * sizeof(double), hipMemcpyDeviceToHost, c->st));
     return 0;

@@ -55,7 +55,8 @@ template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
                            double* partial, long pstride, hipStream_t st);
-void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg, const double* partial,
+// W of one R (index r) from the partials of its (R,T) launches
+void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);
 
 }  // namespace wl

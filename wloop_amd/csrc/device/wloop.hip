@@ -629,24 +629,24 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
   }
 }
 
-// W_ij(R,T;x0) from the partials of every (R,T) launch, in wl_idx order.
-// partial layout per (R,T): [x0][slot][i*msl+j] (nwg slots per x0); the slot
-// sums are added in fixed order with Neumaier compensation, then normalised.
-__global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg,
-                                                    const double* __restrict__ partial,
+// W_ij(R,T;x0) of one R from the partials of its (R,T) launches, stored in
+// wl_idx order.  partial layout: [t][x0][slot][i*msl+j] (nwg slots per x0);
+// the slot sums are added in fixed order with Neumaier compensation, then
+// normalised.
+__global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, int mwlr, int r,
+                                                    int nwg, const double* __restrict__ partial,
                                                     double norm, double* __restrict__ wl) {
   long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long nout = (long)msl * msl * mwlt * mwlr * N0;
+  long nout = (long)msl * msl * mwlt * N0;
   if (id >= nout) return;
-  // id enumerates wl_idx order: i + msl*(j + msl*(t + mwlt*(r + mwlr*x0)))
+  // id enumerates (i, j, t, x0) of this r
   long r_ = id;
   int i = (int)(r_ % msl); r_ /= msl;
   int j = (int)(r_ % msl); r_ /= msl;
   int t = (int)(r_ % mwlt); r_ /= mwlt;
-  int r = (int)(r_ % mwlr); r_ /= mwlr;
   int x0 = (int)r_;
   const int mm = msl * msl;
-  const double* p = partial + (((long)(r * mwlt + t) * N0 + x0) * nwg) * mm + i * msl + j;
+  const double* p = partial + (((long)t * N0 + x0) * nwg) * mm + i * msl + j;
   double sum = 0.0, comp = 0.0;
   for (int w = 0; w < nwg; w++) {
     double v = p[(long)w * mm];
@@ -654,7 +654,8 @@ __global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, i
     comp += (fabs(sum) >= fabs(v)) ? (sum - t2) + v : (v - t2) + sum;
     sum = t2;
   }
-  wl[id] = (sum + comp) / norm;
+  // wl_idx order: i + msl*(j + msl*(t + mwlt*(r + mwlr*x0)))
+  wl[i + (long)msl * (j + (long)msl * (t + (long)mwlt * (r + (long)mwlr * x0)))] = (sum + comp) / norm;
 }
 
 // ---- launchers ------------------------------------------------------------
@@ -799,10 +800,10 @@ void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
   }
 }
 
-void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int nwg, const double* partial,
+void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st) {
-  long nout = (long)msl * msl * mwlt * mwlr * N0;
-  hipLaunchKernelGGL(finish_wloop, grid_for(nout, 1), dim3(256), 0, st, N0, msl, mwlt, mwlr, nwg,
+  long nout = (long)msl * msl * mwlt * N0;
+  hipLaunchKernelGGL(finish_wloop, grid_for(nout, 1), dim3(256), 0, st, N0, msl, mwlt, mwlr, r, nwg,
                      partial, norm, wl);
 }
 
