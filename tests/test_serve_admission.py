@@ -4,7 +4,8 @@ import time
 
 import pytest
 
-from wloop_amd.serve import Admission
+import wloop_amd as W
+from wloop_amd.serve import Admission, Scheduler, expected_ms, standard_request
 
 
 class H:
@@ -70,3 +71,66 @@ def test_abandon_returns_bytes():
     a.acquire("A", 70, close)
     a.abandon(70)
     assert a.used == 0
+
+
+def _req(rid, L, t_arrival, precision=None):
+    r = standard_request(rid, L, 3, W.MIXED if precision is None else precision)
+    r.t_arrival = t_arrival
+    return r
+
+
+def test_expected_cost_grows_with_size_and_precision():
+    costs = [expected_ms(_req(i, L, 0.0)) for i, L in enumerate((12, 16, 24, 32, 48, 64))]
+    assert costs == sorted(costs)
+    assert 500 < costs[4] < 2000  # 48^3x96, mixed: ~1 s measured
+    assert expected_ms(_req(9, 48, 0.0, W.FP64)) > costs[4]
+
+
+def test_fifo_keeps_arrival_order():
+    s = Scheduler("fifo", lambda key: True)
+    now = time.perf_counter()
+    for i, L in enumerate((64, 12, 32)):
+        s.put(_req(i, L, now))
+    assert [s.get().rid for _ in range(3)] == [0, 1, 2]
+
+
+def test_sjf_serves_least_cost_first_and_ages():
+    s = Scheduler("sjf", lambda key: True, age_weight=0.05)
+    now = time.perf_counter()
+    s.put(_req(0, 64, now))
+    s.put(_req(1, 12, now))
+    s.put(_req(2, 32, now))
+    assert [s.get().rid for _ in range(3)] == [1, 2, 0]
+    # a large request that has waited long enough goes before a fresh small one
+    big = _req(3, 64, now - 1000.0)
+    s.put(big)
+    s.put(_req(4, 12, now))
+    assert s.get() is big
+
+
+def test_sjf_prefers_a_shape_with_a_resident_context(monkeypatch):
+    monkeypatch.setattr(W, "footprint", lambda dims, parms, prec: 200e9)
+    resident = {((32, 16, 16, 16), 3, W.MIXED)}
+    s = Scheduler("sjf", lambda key: key in resident, create_ms_per_gb=10.0)
+    now = time.perf_counter()
+    s.put(_req(0, 12, now))   # cheaper, but its context would be created
+    s.put(_req(1, 16, now))   # resident
+    assert s.get().rid == 1
+
+
+def test_join_waits_for_task_done():
+    s = Scheduler("fifo", lambda key: True)
+    s.put(_req(0, 12, time.perf_counter()))
+    r = s.get()
+    done = []
+
+    def finish():
+        time.sleep(0.05)
+        done.append(r.rid)
+        s.task_done()
+
+    t = threading.Thread(target=finish)
+    t.start()
+    s.join()
+    t.join()
+    assert done == [0]

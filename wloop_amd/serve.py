@@ -11,7 +11,9 @@ measurement (HYP, operator levels, transporters, W_ij(R,T;x0)) there.
   context needs (wlg_footprint) fit in the budget next to the contexts that
   already exist; idle contexts are evicted least-recently-used first; a
   request that cannot fit even on an empty device is rejected.
-* Workers run requests concurrently, each context on its own HIP stream.
+* Workers run requests concurrently, each context on its own HIP stream; the
+  next request is the one of least predicted cost (with an ageing credit and
+  a bonus for a resident context of its shape), or the oldest with --policy fifo.
 * Reported: p50/p99 latency (arrival to W on the host), configs/s, service
   times, rejections.
 
@@ -20,7 +22,6 @@ measurement (HYP, operator levels, transporters, W_ij(R,T;x0)) there.
 import argparse
 import collections
 import json
-import queue
 import random
 import sys
 import threading
@@ -87,6 +88,10 @@ class Admission:
                     return None, True
                 self.cv.wait()
 
+    def has_idle(self, key):
+        with self.cv:
+            return any(k == key for k, _ in self.idle.values())
+
     def register(self, handle, key, nbytes):
         with self.cv:
             self.busy[handle] = (key, nbytes)
@@ -103,11 +108,77 @@ class Admission:
             self.cv.notify_all()
 
 
+def expected_ms(req):
+    """Predicted service time of a request on one MI355X (ms): per-site costs of
+    HYP and the transporters, of an APE step and of one (R,T) contraction,
+    measured at 48^3x96 (mixed path; the FP64 path is about twice as slow),
+    plus a fixed launch overhead."""
+    V = float(np.prod(req.dims))
+    p = req.parms
+    steps = sum(p.nss[:p.msl])
+    ns_per_site = 15.0 + 0.29 * steps + 0.2 * p.mwlr * p.mwlt
+    return 20.0 + V * ns_per_site * 1e-6 * (1.0 if req.precision == W.MIXED else 2.0)
+
+
+class Scheduler:
+    """Pending requests.  "fifo" serves them in arrival order; "sjf" serves the
+    request with the least predicted cost: expected_ms, plus the cost of
+    creating its context when no idle context of its shape is resident, minus
+    an ageing credit of `age_weight` ms per ms waited (no request waits
+    forever)."""
+
+    def __init__(self, policy, resident, age_weight=0.05, create_ms_per_gb=2.0):
+        self.policy, self.resident = policy, resident
+        self.age_weight, self.create_ms_per_gb = age_weight, create_ms_per_gb
+        self.pending = []
+        self.cv = threading.Condition()
+        self.open_items = 0
+
+    def put(self, req):
+        with self.cv:
+            self.pending.append(req)
+            self.open_items += 1
+            self.cv.notify()
+
+    def _score(self, req, now):
+        cost = expected_ms(req)
+        if not self.resident((req.dims, req.bc, req.precision)):
+            cost += 20.0 + self.create_ms_per_gb * W.footprint(req.dims, req.parms, req.precision) / 1e9
+        return cost - self.age_weight * (now - req.t_arrival) * 1e3
+
+    def get(self):
+        with self.cv:
+            while not self.pending:
+                self.cv.wait()
+            if self.policy == "fifo" or self.pending[0] is None:
+                return self.pending.pop(0)
+            now = time.perf_counter()
+            live = [r for r in self.pending if r is not None]
+            best = min(live, key=lambda r: self._score(r, now))
+            self.pending.remove(best)
+            return best
+
+    def stop_one(self):
+        with self.cv:
+            self.pending.append(None)
+            self.cv.notify()
+
+    def task_done(self):
+        with self.cv:
+            self.open_items -= 1
+            self.cv.notify_all()
+
+    def join(self):
+        with self.cv:
+            while self.open_items > 0:
+                self.cv.wait()
+
+
 class Server:
-    def __init__(self, budget_bytes, workers=2, device=0):
+    def __init__(self, budget_bytes, workers=2, device=0, policy="sjf"):
         self.adm = Admission(budget_bytes)
         self.device = device
-        self.q = queue.Queue()
+        self.q = Scheduler(policy, self.adm.has_idle)
         self.done = []
         self.lock = threading.Lock()
         self.threads = [threading.Thread(target=self._worker, daemon=True) for _ in range(workers)]
@@ -122,7 +193,6 @@ class Server:
         while True:
             req = self.q.get()
             if req is None:
-                self.q.task_done()
                 return
             try:
                 self._serve(req)
@@ -156,7 +226,7 @@ class Server:
 
     def stop(self):
         for _ in self.threads:
-            self.q.put(None)
+            self.q.stop_one()
         for t in self.threads:
             t.join()
         for h in list(self.adm.idle):
@@ -179,11 +249,11 @@ def summarize(reqs, t0):
             "errors": sorted({r.error for r in reqs if r.error})[:5]}
 
 
-def run_stream(rate, n, precision, budget_gb, workers=2, sizes=SIZES, seed=0, log=None):
+def run_stream(rate, n, precision, budget_gb, workers=2, sizes=SIZES, seed=0, log=None, policy="sjf"):
     """Poisson stream of n requests (mean `rate` per second); sizes uniform over
     `sizes`, boundary conditions half open, half periodic."""
     rng = random.Random(seed)
-    srv = Server(int(budget_gb * 1e9), workers)
+    srv = Server(int(budget_gb * 1e9), workers, policy=policy)
     reqs = []
     t0 = time.perf_counter()
     for i in range(n):
@@ -206,7 +276,7 @@ def run_stream(rate, n, precision, budget_gb, workers=2, sizes=SIZES, seed=0, lo
     srv.drain()
     out = summarize(reqs, t0)
     out.update({"offered_rate_per_s": rate, "precision": "mixed" if precision == W.MIXED else "fp64",
-                "budget_gb": budget_gb, "workers": workers, "sizes": list(sizes),
+                "budget_gb": budget_gb, "workers": workers, "sizes": list(sizes), "policy": policy,
                 "evicted_contexts": srv.adm.evicted})
     srv.stop()
     return out, reqs
@@ -220,11 +290,13 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=260.0)
     ap.add_argument("--workers", type=int, default=2)
     ap.add_argument("--sizes", default=",".join(map(str, SIZES)))
+    ap.add_argument("--policy", choices=("sjf", "fifo"), default="sjf",
+                    help="sjf: least predicted cost first, with ageing; fifo: arrival order")
     args = ap.parse_args()
     sizes = tuple(int(s) for s in args.sizes.split(","))
     prec = W.MIXED if args.precision == "mixed" else W.FP64
     out, _ = run_stream(args.rate, args.requests, prec, args.budget_gb, args.workers, sizes,
-                        log=lambda *a: print(*a, file=sys.stderr, flush=True))
+                        log=lambda *a: print(*a, file=sys.stderr, flush=True), policy=args.policy)
     print(json.dumps(out))
 
 
