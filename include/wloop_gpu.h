@@ -119,6 +119,7 @@ enum {
    WLG_TUNE_MIXED_CONTRACT_KERNEL, WLG_TUNE_MIXED_CONTRACT_WG,
    WLG_TUNE_FP64_CONTRACT_DEPTH, WLG_TUNE_FP64_CONTRACT_STARTS,
    WLG_TUNE_MIXED_CONTRACT_DEPTH, WLG_TUNE_MIXED_CONTRACT_STARTS,
+   WLG_TUNE_APE_FORM,   /* 1: one kernel per APE step, 2: staple sums, then lincomb and projection */
    WLG_TUNE_COUNT
 };
 int wlg_tune(wlg_ctx *ctx, int param, int value);

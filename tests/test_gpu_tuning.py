@@ -86,3 +86,22 @@ def test_autotune_picks_a_candidate(wl, tmp_path):
         assert lat.tune_get(wl.TUNE_FP64_CONTRACT_KERNEL) == ch["kernel"]
         again = autotune.tune(lat, p, wl.FP64, cache=cache)
         assert again["cached"] and again["kernel"] == ch["kernel"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_ape_forms_agree(wl, precision):
+    """APE as one kernel or as staple sums then projection: the same arithmetic,
+    so the same bits."""
+    dims, bc = (8, 6, 5, 7), 0
+    u = O.generate(dims, bc, 46, 0.3)
+    nss = [0, 2, 3]
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, 4, 2))
+    p = wl.make_parms(nss, 4, 2)
+    outs = []
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        for form in (1, 2):
+            lat.tune(wl.TUNE_APE_FORM, form)
+            outs.append(lat.measure(p, precision))
+    assert np.array_equal(outs[0], outs[1])
+    assert rel_err(outs[1], ref) < (1e-12 if precision == 0 else 1e-6)

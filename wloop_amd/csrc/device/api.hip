@@ -368,7 +368,7 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64)
   size_t off = 0;
   const size_t sl = (prec == WLG_MIXED) ? s32 : s64;  // storage of levels and transporters
   o.lev = off; off += (size_t)3 * p->msl * sl;
-  o.tmp = off; off += (p->op_smear == 1 ? 12 : 6) * sl;
+  o.tmp = off; off += (p->op_smear == 1 ? 12 : 9) * sl;  // APE: 2x3 ping-pong + 3 staple sums
   if (prec == WLG_MIXED) {
     o.src32 = off; off += 3 * s32;
     o.lt32 = off; off += (size_t)p->mwlt * s32;
@@ -447,7 +447,7 @@ static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, char*
   v2<TS>* levbuf = (v2<TS>*)levbase;
   v2<TS>* tmp0 = (v2<TS>*)tmpbase;
   v2<TS>* tmp1 = tmp0 + lvl;
-  v2<TS>* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots)
+  v2<TS>* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots) / APE staple sums (3)
   for (int sl = 0; sl < msl; sl++) {
     const v2<TS>* cur = (sl == 0) ? src : lev[sl - 1];
     int n = p->nss[sl];
@@ -461,7 +461,10 @@ static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, char*
         launch_hyp3<TS, TS>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
                             p->proj_iter, c->st);
       else
-        launch_ape<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+        if (c->tune[WLG_TUNE_APE_FORM] == 2)
+          launch_ape_split<TS, TS>(L, cur, tmpD, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+        else
+          launch_ape<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
       CKL();
       cur = dst;
     }
@@ -831,6 +834,8 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if ((param == WLG_TUNE_FP64_CONTRACT_STARTS || param == WLG_TUNE_MIXED_CONTRACT_STARTS) &&
       value != 0 && value != 4 && value != 8)
     return set_err("start slices per workgroup must be 0 (default), 4 or 8");
+  if (param == WLG_TUNE_APE_FORM && value > 2)
+    return set_err("APE form must be 0 (default), 1 (one kernel) or 2 (staples, then projection)");
   if ((param == WLG_TUNE_FP64_CONTRACT_WG || param == WLG_TUNE_MIXED_CONTRACT_WG) && value > 65536)
     return set_err("workgroup parameter must be 0 (default) or 1..65536");
   c->tune[param] = value;

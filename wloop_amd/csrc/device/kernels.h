@@ -17,6 +17,11 @@ void launch_hyp3(const Lat& L, const v2<TS>* F, v2<TS>* D, v2<TS>* out, double a
 template <class TC, class TS>
 void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
                 hipStream_t st);
+// APE step as a staple-sum kernel (sums into S, 3 slots of TC) and a
+// lincomb/projection kernel
+template <class TC, class TS>
+void launch_ape_split(const Lat& L, const v2<TS>* in, v2<TC>* S, v2<TS>* out, double alpha, int proj,
+                      int piter, hipStream_t st);
 void launch_ape4(const Lat& L, const double2* in, double2* out, double alpha, int proj, int piter,
                  hipStream_t st);
 void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2* v, int mu, int nu,

@@ -198,6 +198,41 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2
   decorate_store<TC, TS>(S, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
 }
 
+// The APE step in two kernels: the staple sums of every spatial link (in TC,
+// stored in TC), then the FP64 lincomb and projection.  The first is light in
+// registers and bound by link-load latency, the second is dense FP64 work on
+// two streamed links; each runs at the occupancy its own register use allows.
+// Same arithmetic as ape_step.
+template <class TC, class TS>
+__global__ __launch_bounds__(256, sizeof(TC) == 4 ? 4 : 2) void ape_staples(Lat L, const v2<TS>* __restrict__ in,
+                                                      v2<TC>* __restrict__ S) {
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = 1 + blockIdx.y;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> v;
+  set_zero(v);
+  for (int eta = 1; eta < 4; eta++) {
+    if (eta == mu) continue;
+    staple_add<TC, TS>(v, L, s, x, in, eta - 1, in, mu - 1, mu, eta);
+  }
+  store(v, S, L.V, mu - 1, s);
+}
+
+template <class TC, class TS>
+__global__ __launch_bounds__(256, 3) void ape_project(Lat L, const v2<TS>* __restrict__ in,
+                                                      const v2<TC>* __restrict__ S,
+                                                      v2<TS>* __restrict__ out, double alpha,
+                                                      int proj, int piter) {
+  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= L.V) return;
+  int mu = 1 + blockIdx.y;
+  M3<TC> v;
+  load(v, S, L.V, mu - 1, s);
+  decorate_store<TC, TS>(v, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
+}
+
 // 4-d APE step (ape_links.c with time = 1) on a 4-slot field.
 template <class TC, class TS>
 __global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict__ in,
@@ -265,6 +300,14 @@ void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int p
                      proj, piter);
 }
 
+template <class TC, class TS>
+void launch_ape_split(const Lat& L, const v2<TS>* in, v2<TC>* S, v2<TS>* out, double alpha, int proj,
+                      int piter, hipStream_t st) {
+  hipLaunchKernelGGL((ape_staples<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, S);
+  hipLaunchKernelGGL((ape_project<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, S, out, alpha,
+                     proj, piter);
+}
+
 void launch_ape4(const Lat& L, const double2* in, double2* out, double alpha, int proj, int piter,
                  hipStream_t st) {
   hipLaunchKernelGGL((ape_step4<double, double>), grid_for(L.V, 4), dim3(256), 0, st, L, in, out,
@@ -290,6 +333,10 @@ template void launch_ape<float, double>(const Lat&, const double2*, double2*, do
                                         hipStream_t);
 template void launch_ape<float, float>(const Lat&, const float2*, float2*, double, int, int,
                                        hipStream_t);
+template void launch_ape_split<double, double>(const Lat&, const double2*, double2*, double2*, double,
+                                               int, int, hipStream_t);
+template void launch_ape_split<float, float>(const Lat&, const float2*, float2*, float2*, double, int,
+                                             int, hipStream_t);
 template void launch_hyp3<float, float>(const Lat&, const float2*, float2*, float2*, double, double,
                                         int, int, hipStream_t);
 
