@@ -290,7 +290,7 @@ def main():
                 "algorithmic_bytes_per_launch": algo}
 
     k_mx = choice["mixed"]["kernel"] if choice else W.KERNEL_CHAIN1
-    k_64 = plain["kernel"] if choice else W.KERNEL_STREAM
+    k_64 = plain["kernel"] if choice else W.KERNEL_CHAIN2
     roof_mx = roof(csmx, "mixed", k_mx)
     roof_64 = roof(cs64, "fp64", k_64)
     if best64:

@@ -108,7 +108,9 @@ size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
 /* launch-shape parameters of a context (0 = library default).  For each
  * precision: the contraction kernel (1 streaming form, 2 chain form with msl
  * lanes per (site,k), 3 chain form with one lane per (site,k), msl <= 4,
- * 4 fused time-slice form, msl <= 4) and its workgroup parameter (chunks per
+ * 4 fused time-slice form, msl <= 4, 5 chain form with the levels of a
+ * (site,k) split over two lanes, msl = 2 or 4, else the one-lane form) and
+ * its workgroup parameter (chunks per
  * (x0,k) for the streaming form, persistent workgroups for the chain forms,
  * workgroups per group of start slices for the fused form); for the fused
  * form also the time-slice depth (T values per launch, 1..16) and the start

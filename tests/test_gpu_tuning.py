@@ -20,7 +20,8 @@ def test_contraction_variants_agree(wl, precision):
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
         for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
-                            (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40))):
+                            (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40)),
+                            (wl.KERNEL_CHAIN2, (1, 6, 512))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)
@@ -63,9 +64,37 @@ def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision):
             assert rel_err(w, base) < (1e-13 if precision == 0 else 1e-7), (depth, starts, wg)
 
 
+# two-lane chain form: msl 2 and 4 (msl 3 runs the one-lane form), a spatial
+# slice that is not a multiple of the 128-site block, chains of several
+# lengths (T against N0 = 10), open and periodic boundaries.  Each trace has
+# the bits of the one-lane form's trace; only the FP64 summation order differs.
+@pytest.mark.parametrize("nss", [[0, 2], [0, 1, 3], [0, 2, 2, 5]])
+@pytest.mark.parametrize("bc", [0, 3])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_two_lane_chain(wl, nss, bc, precision):
+    dims, mwlt, mwlr = (10, 5, 6, 7), 6, 3
+    u = O.generate(dims, bc, 47, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    pk = wl.TUNE_MIXED_CONTRACT_KERNEL if precision else wl.TUNE_FP64_CONTRACT_KERNEL
+    tol = 1e-12 if precision == 0 else 1e-6
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        lat.tune(pk, wl.KERNEL_CHAIN1)
+        lat.tune(pk + 1, 7)
+        one = lat.measure(p, precision)
+        lat.tune(pk, wl.KERNEL_CHAIN2)
+        for wg in (1, 3, 64):
+            lat.tune(pk + 1, wg)
+            print(f"[two-lane] nss={nss} bc={bc} prec={precision} wg={wg}", flush=True)
+            w = lat.measure(p, precision)
+            assert rel_err(w, ref) < tol, wg
+            assert rel_err(w, one) < 1e-11, wg
+
+
 def test_fused_rejects_bad_shapes(wl):
     with wl.Lattice((8, 4, 4, 4), 3) as lat:
-        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 5), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 6), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
                              (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
             with pytest.raises(wl.WloopError):
                 lat.tune(param, value)

@@ -19,6 +19,7 @@ def _cands():
     out = [{"kernel": W.KERNEL_STREAM, "wg": wg} for wg in (4, 8, 16, 32)]
     out += [{"kernel": W.KERNEL_CHAIN, "wg": wg} for wg in (512, 2048)]
     out += [{"kernel": W.KERNEL_CHAIN1, "wg": wg} for wg in (256, 512, 1024)]
+    out += [{"kernel": W.KERNEL_CHAIN2, "wg": wg} for wg in (256, 512, 1024)]
     out += [{"kernel": W.KERNEL_DEPTH, "wg": wg, "depth": d, "starts": st}
             for d in (8, 16) for st in (4, 8) for wg in (32, 64, 128)]
     return out

@@ -394,11 +394,11 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int td = t[mixed ? WLG_TUNE_MIXED_CONTRACT_DEPTH : WLG_TUNE_FP64_CONTRACT_DEPTH];
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
-  // defaults (measured at 48^3x96): streaming form for FP64, one-lane chain
-  // form for the FP32 transporters of the mixed path
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 0);
+  // defaults (measured at 48^3x96): two-lane chain form for FP64, one-lane
+  // chain form for the FP32 transporters of the mixed path
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 4);
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[4] = {8, 512, 512, 128};  // per form, measured at 48^3x96
+  static const int wg_default[5] = {8, 512, 512, 128, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : 8;
@@ -825,9 +825,10 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 4)
+      value > 5)
     return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
-                   "per pair), 3 (chain, one lane per pair) or 4 (fused time slices)");
+                   "per pair), 3 (chain, one lane per pair), 4 (fused time slices) or 5 (chain, "
+                   "two lanes per pair)");
   if ((param == WLG_TUNE_FP64_CONTRACT_DEPTH || param == WLG_TUNE_MIXED_CONTRACT_DEPTH) &&
       value > 16)
     return set_err("time-slice depth must be 0 (default) or 1..16");

@@ -380,6 +380,187 @@ template <int CTRL> __device__ __forceinline__ double row_dpp(double v) {
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ double lane_xor1(double v) { return row_dpp<kQuadXor1>(v); }
+__device__ __forceinline__ float lane_xor1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kQuadXor1, 0xF, 0xF, false));
+}
+
+// ---- contraction, two-lane chain form ------------------------------------
+// As the one-lane chain form, but the msl levels of a (site, k) pair are split
+// over two neighbouring lanes: lane 2p+h carries D_i and loads the sink links
+// S_j of the levels h MH .. h MH + MH-1 (MH = msl/2), which halves the
+// registers per lane (two waves per SIMD in FP64 instead of one).  The traces
+// with the partner's sink links take them one row at a time through a DPP
+// exchange of lanes 2p and 2p+1, in the operation order of retr_dag_mul, so
+// every trace has the bits the one-lane form gives it.  Every link is still
+// read once per (R,T).  Work units are blocks of 128 sites of one direction k
+// (32 pairs per wave), the three directions of a block next to each other,
+// each XCD on a contiguous range of units.  The traces of a step are summed
+// over the pairs of a wave as in the one-lane chain form; with WACC each wave
+// adds them into its own LDS accumulator (the waves are combined in fixed
+// order at the end), so the waves of a workgroup do not wait for each other
+// at every step (4.0 against 4.2 ms per (R,T) in FP64 at 48^3x96).  Requires
+// msl = 2 or 4.
+template <int MSL, class TS, class TC, bool WACC>
+__global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, int g,
+                                                          const v2<TS>* __restrict__ LT,
+                                                          LevelPtrs<TS> S,
+                                                          double* __restrict__ partial) {
+  constexpr int MM = MSL * MSL;
+  constexpr int MH = MSL / 2;
+  static_assert(MSL == 2 || MSL == 4, "two-lane chain form needs msl = 2 or 4");
+  extern __shared__ double lds_c2[];
+  const int N0 = L.n[0];
+  // WACC: one accumulator per wave, [4][N0][MM], and no workgroup barrier per
+  // step; else one per workgroup, [N0][MM], with the wave sums staged in red2
+  constexpr int NACC = WACC ? 4 : 1;
+  double* acc = lds_c2;                                   // [NACC][N0][MM]
+  double* red2 = lds_c2 + NACC * N0 * MM;                 // [2][4][16]
+  TC* redf = (TC*)(red2 + 2 * 4 * 16);                    // [4][32][16], traces in TC
+  const int len = N0 / g;
+  const long slice = L.st[0];
+  const long nbk = (slice + 127) / 128;  // blocks of 128 sites per direction
+  const long nblocks = 3 * nbk;
+  const long units = nblocks * g;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane & 1, pr = lane >> 1;
+  TC* rw = redf + wave * 32 * 16;
+  // this lane's levels i = h MH + m
+  const v2<TS>* sp[MH];
+#pragma unroll
+  for (int m = 0; m < MH; m++) sp[m] = S.p[h * MH + m];
+
+  for (int q = threadIdx.x; q < NACC * N0 * MM; q += 256) acc[q] = 0.0;
+  __syncthreads();
+  double* accw = acc + (WACC ? wave * N0 * MM : 0);
+  int par = 0;
+  for (long u = xcd_order(blockIdx.x, gridDim.x); u < units; u += gridDim.x) {
+    const int c = (int)(u / nblocks);
+    const long bl = u % nblocks;
+    const int k = (int)(bl % 3);
+    const long xs0 = (bl / 3) * 128 + (threadIdx.x >> 1);
+    const bool live = xs0 < slice;
+    const long xs = live ? xs0 : 0;
+    int x[4];
+    coords(L, xs, x);
+    M3<TC> D[MH], Sx[MH];
+    {
+      const long s0 = (long)c * slice + xs;
+      M3<TS> a, b, t;
+      load(a, LT, L.V, 0, s0);
+      load(b, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
+      M3<TC> ac = convert<TC>(a), bc = convert<TC>(b);
+#pragma unroll
+      for (int m = 0; m < MH; m++) {
+        load(t, sp[m], L.V, k, s0);
+        D[m] = mul(dag_mul(ac, convert<TC>(t)), bc);
+      }
+    }
+    int x0 = c;
+    int x1 = x0 + T;
+    if (x1 >= N0) x1 -= N0;
+#pragma unroll
+    for (int m = 0; m < MH; m++) {
+      M3<TS> t;
+      load(t, sp[m], L.V, k, (long)x1 * slice + xs);
+      Sx[m] = convert<TC>(t);
+    }
+    for (int n = 0; n < len; n++) {
+      // traces with this lane's sink links, then with the partner's
+#pragma unroll
+      for (int mi = 0; mi < MH; mi++) {
+        const int i = h * MH + mi;
+#pragma unroll
+        for (int mj = 0; mj < MH; mj++) {
+          const int j = h * MH + mj;
+          rw[pr * 16 + i * MSL + j] = live ? retr_dag_mul(Sx[mj], D[mi]) : TC(0);
+        }
+      }
+#pragma unroll
+      for (int mj = 0; mj < MH; mj++) {
+        TC rp[MH][3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          TC pre[3], pim[3];
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            pre[e] = lane_xor1(Sx[mj].re[3 * r + e]);
+            pim[e] = lane_xor1(Sx[mj].im[3 * r + e]);
+          }
+#pragma unroll
+          for (int mi = 0; mi < MH; mi++) {
+            TC t = pre[0] * D[mi].re[3 * r];
+            t = fmadd(pim[0], D[mi].im[3 * r], t);
+#pragma unroll
+            for (int e = 1; e < 3; e++) {
+              t = fmadd(pre[e], D[mi].re[3 * r + e], t);
+              t = fmadd(pim[e], D[mi].im[3 * r + e], t);
+            }
+            rp[mi][r] = t;
+          }
+        }
+        const int j = (1 - h) * MH + mj;
+#pragma unroll
+        for (int mi = 0; mi < MH; mi++) {
+          const int i = h * MH + mi;
+          rw[pr * 16 + i * MSL + j] = live ? rp[mi][0] + rp[mi][1] + rp[mi][2] : TC(0);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      {
+        const int q = lane & 15, grp = lane >> 4;
+        double sred = 0.0;
+        if (q < MM) {
+#pragma unroll
+          for (int m = 0; m < 8; m++) sred += (double)rw[(grp * 8 + m) * 16 + q];
+        }
+        sred += __shfl_xor(sred, 16, 64);
+        sred += __shfl_xor(sred, 32, 64);
+        if constexpr (WACC) {
+          if (lane < MM) accw[x0 * MM + lane] += sred;
+        } else {
+          if (lane < 16) red2[(par * 4 + wave) * 16 + lane] = sred;
+        }
+      }
+      if constexpr (!WACC) {
+        __syncthreads();
+        if (threadIdx.x < MM) {
+          const double* r2 = red2 + par * 64;
+          acc[x0 * MM + threadIdx.x] += ((r2[threadIdx.x] + r2[16 + threadIdx.x]) + r2[32 + threadIdx.x]) +
+                                        r2[48 + threadIdx.x];
+        }
+        par ^= 1;
+      }
+      if (n + 1 < len) {
+        // next source slice x1: D_i = L(x1)^dag S_i(x1) L(x1+Rk)
+        const long s1 = (long)x1 * slice + xs;
+        M3<TS> l1, l1r;
+        load(l1, LT, L.V, 0, s1);
+        load(l1r, LT, L.V, 0, fwd(L, s1, x, k + 1, R));
+        M3<TC> a = convert<TC>(l1), b = convert<TC>(l1r);
+#pragma unroll
+        for (int m = 0; m < MH; m++) D[m] = mul(dag_mul(a, Sx[m]), b);
+        x0 = x1;
+        x1 = x0 + T;
+        if (x1 >= N0) x1 -= N0;
+#pragma unroll
+        for (int m = 0; m < MH; m++) {
+          M3<TS> t;
+          load(t, sp[m], L.V, k, (long)x1 * slice + xs);
+          Sx[m] = convert<TC>(t);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N0 * MM; q += 256) {
+    const int xo = q / MM, ij = q % MM;
+    double v = acc[q];
+    if constexpr (WACC) v = ((v + acc[N0 * MM + q]) + acc[2 * N0 * MM + q]) + acc[3 * N0 * MM + q];
+    partial[((long)xo * gridDim.x + blockIdx.x) * MM + ij] = v;
+  }
+}
 
 // ---- contraction, fused time-slice form ----------------------------------
 // One launch covers T = Ta .. Ta+d-1 of one R.  A lane owns (pair p of a block
@@ -720,6 +901,20 @@ static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
                      sp, partial);
 }
 
+template <int MSL, class TS, class TC>
+static void chain2_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st) {
+  // per-wave accumulators where 4 of them fit into LDS (N0 <= 300 at msl 4)
+  const size_t rest = 2 * 4 * 16 * sizeof(double) + 4 * 32 * 16 * sizeof(TC);
+  const size_t acc1 = (size_t)L.n[0] * MSL * MSL * sizeof(double);
+  if (4 * acc1 + rest <= 163840)
+    hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true>), dim3(nwg), dim3(256), 4 * acc1 + rest, st, L,
+                       R, T, g, LT, sp, partial);
+  else
+    hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false>), dim3(nwg), dim3(256), acc1 + rest, st, L, R,
+                       T, g, LT, sp, partial);
+}
+
 size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes) {
   const size_t slot = (size_t)msl * 9 * 16 * 2 * ts_bytes + (size_t)2 * 9 * 16 * 2 * tc_bytes;
   return (size_t)W * 256 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot;
@@ -761,7 +956,9 @@ void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, i
 
 // kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl
 // lanes per pair) with wg persistent workgroups; 2: one-lane chain form with
-// wg persistent workgroups (msl <= 4; otherwise the streaming form).
+// wg persistent workgroups (msl <= 4; otherwise the streaming form); 4:
+// two-lane chain form with wg persistent workgroups (msl = 2 or 4; the
+// one-lane form for msl = 1 or 3, the streaming form for msl > 4).
 // partial receives contract_slots() sums per x0.
 template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
@@ -774,7 +971,10 @@ void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
     g = t;
     t = r;
   }
-  if (kernel == 2 && msl <= 4) {
+  if (kernel == 4 && (msl == 2 || msl == 4)) {
+    if (msl == 2) chain2_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st);
+    else chain2_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st);
+  } else if ((kernel == 2 || kernel == 4) && msl <= 4) {
     switch (msl) {
       case 1: chain1_launch<1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;
       case 2: chain1_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st); break;

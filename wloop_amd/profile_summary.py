@@ -19,7 +19,8 @@ import os
 import re
 import sys
 
-FORMS = (("contract_stream", 1), ("contract_chain1", 3), ("contract_chain", 2), ("contract_depth", 4))
+FORMS = (("contract_stream", 1), ("contract_chain1", 3), ("contract_chain2", 5), ("contract_chain", 2),
+         ("contract_depth", 4))
 
 
 def _csv(d, suffix):
