@@ -283,13 +283,26 @@ def main():
                     "algorithmic_flop_per_launch": flop}
         link_bytes = 72 if prec_name == "mixed" else 144
         algo = V * links_per_site * link_bytes
+        if kernel == W.KERNEL_HYBRID and prec_name == "mixed":
+            # two launch kinds (one fused launch per R and block of T <= 8, one
+            # one-lane chain launch per (R,T) beyond): the unit is one (R,T) pair
+            n_rt = args.steps * parms.mwlt * parms.mwlr
+            per_rt = ms / n_rt / 1e3
+            nf = min(parms.mwlt, 8)
+            tf, tc = traffic.get(f"{W.KERNEL_DEPTH}/mixed"), traffic.get(f"{W.KERNEL_CHAIN1}/mixed")
+            tr = (tf + tc * (parms.mwlt - nf)) / parms.mwlt if tf and tc else None
+            ach = algo / per_rt / 1e9
+            return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": ach / HBM_PEAK_GBS, "traffic": tr, "kernel": W.KERNEL_NAMES[kernel],
+                    "basis": "per (R,T) pair over both launch kinds", "avg_ms_per_rt": per_rt * 1e3,
+                    "algorithmic_bytes_per_rt": algo}
         ach = algo / avg_s / 1e9
         return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(f"{kernel}/{prec_name}"),
                 "kernel": W.KERNEL_NAMES[kernel], "avg_launch_ms": avg_s * 1e3,
                 "algorithmic_bytes_per_launch": algo}
 
-    k_mx = choice["mixed"]["kernel"] if choice else W.KERNEL_CHAIN1
+    k_mx = choice["mixed"]["kernel"] if choice else W.KERNEL_HYBRID
     k_64 = plain["kernel"] if choice else W.KERNEL_CHAIN2
     roof_mx = roof(csmx, "mixed", k_mx)
     roof_64 = roof(cs64, "fp64", k_64)

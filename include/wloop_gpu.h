@@ -109,12 +109,15 @@ size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
  * precision: the contraction kernel (1 streaming form, 2 chain form with msl
  * lanes per (site,k), 3 chain form with one lane per (site,k), msl <= 4,
  * 4 fused time-slice form, msl <= 4, 5 chain form with the levels of a
- * (site,k) split over two lanes, msl = 2 or 4, else the one-lane form) and
+ * (site,k) split over two lanes, msl = 2 or 4, else the one-lane form, 6 the
+ * fused form for the FP32 T blocks (T <= 8) of the mixed path and the one-lane
+ * chain form for the T beyond, msl <= 4; on the FP64 path form 6 is form 5) and
  * its workgroup parameter (chunks per
  * (x0,k) for the streaming form, persistent workgroups for the chain forms,
- * workgroups per group of start slices for the fused form); for the fused
- * form also the time-slice depth (T values per launch, 1..16) and the start
- * slices per workgroup (4 or 8; one wave each).  Results do not depend on
+ * workgroups per group of start slices for the fused form; form 6 uses it for
+ * both of its launch kinds); for the fused forms also the time-slice depth (T
+ * values per launch, 1..16) and the start slices per workgroup (4 or 8; one
+ * wave each).  Results do not depend on
  * them beyond floating-point rounding and summation order. */
 enum {
    WLG_TUNE_FP64_CONTRACT_KERNEL = 0, WLG_TUNE_FP64_CONTRACT_WG,

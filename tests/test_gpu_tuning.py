@@ -21,7 +21,7 @@ def test_contraction_variants_agree(wl, precision):
         lat.upload(u)
         for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
                             (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40)),
-                            (wl.KERNEL_CHAIN2, (1, 6, 512))):
+                            (wl.KERNEL_CHAIN2, (1, 6, 512)), (wl.KERNEL_HYBRID, (1, 8, 40))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)
@@ -32,13 +32,16 @@ def test_contraction_variants_agree(wl, precision):
         assert rel_err(w, outs[0]) < (1e-13 if precision == 0 else 1e-7)
 
 
-# fused time-slice form: ragged pair blocks (spatial slice not a multiple of
-# 16), start groups that overrun N0, open and periodic boundaries, and on the
-# mixed path T blocks on both sides of the FP32/FP64 split at T = 9
+# fused time-slice forms (4: every T; 6: the FP32 blocks T <= 8 of the mixed
+# path, the one-lane chain form beyond): ragged pair blocks (spatial slice not
+# a multiple of 16), start groups that overrun N0, open and periodic
+# boundaries, and on the mixed path T blocks on both sides of the FP32/FP64
+# split at T = 9
 @pytest.mark.parametrize("dims,bc,mwlt,mwlr", [((10, 5, 6, 7), 0, 5, 2), ((20, 5, 6, 7), 3, 11, 3),
                                                ((18, 4, 4, 6), 0, 10, 2)])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision):
+@pytest.mark.parametrize("kernel", [4, 6])
+def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel):
     nss = [0, 2, 3]
     u = O.generate(dims, bc, 45, 0.3)
     ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
@@ -52,13 +55,13 @@ def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision):
         lat.upload(u)
         base = lat.measure(p, precision)  # library default form
         assert rel_err(base, ref) < tol
-        lat.tune(pk, wl.KERNEL_DEPTH)
+        lat.tune(pk, kernel)
         for depth, starts, wg in ((16, 4, 8), (3, 4, 5), (1, 8, 1), (8, 8, 16)):
             lat.tune(pd, depth)
             lat.tune(ps, starts)
             lat.tune(pw, wg)
-            print(f"[fused] dims={dims} bc={bc} prec={precision} depth={depth} starts={starts} wg={wg}",
-                  flush=True)
+            print(f"[fused] kernel={kernel} dims={dims} bc={bc} prec={precision} depth={depth} "
+                  f"starts={starts} wg={wg}", flush=True)
             w = lat.measure(p, precision)
             assert rel_err(w, ref) < tol, (depth, starts, wg)
             assert rel_err(w, base) < (1e-13 if precision == 0 else 1e-7), (depth, starts, wg)
@@ -94,7 +97,7 @@ def test_two_lane_chain(wl, nss, bc, precision):
 
 def test_fused_rejects_bad_shapes(wl):
     with wl.Lattice((8, 4, 4, 4), 3) as lat:
-        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 6), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 7), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
                              (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
             with pytest.raises(wl.WloopError):
                 lat.tune(param, value)

@@ -20,10 +20,11 @@ T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
 (TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG,
  TUNE_FP64_CONTRACT_DEPTH, TUNE_FP64_CONTRACT_STARTS, TUNE_MIXED_CONTRACT_DEPTH,
  TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM) = range(9)
-KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH, KERNEL_CHAIN2 = 1, 2, 3, 4, 5
+KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH, KERNEL_CHAIN2, KERNEL_HYBRID = 1, 2, 3, 4, 5, 6
 KERNEL_NAMES = {KERNEL_STREAM: "streaming", KERNEL_CHAIN: "chain (msl lanes per pair)",
                 KERNEL_CHAIN1: "chain (one lane per pair)", KERNEL_DEPTH: "fused time slices",
-                KERNEL_CHAIN2: "chain (two lanes per pair)"}
+                KERNEL_CHAIN2: "chain (two lanes per pair)",
+                KERNEL_HYBRID: "fused time slices for T <= 8, one-lane chain beyond"}
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
 
 

@@ -386,6 +386,18 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64)
 
 static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
 
+// T blocks a contraction form runs with the fused time-slice kernel: every T
+// (form 3) or, for form 5 on the mixed path, the FP32 blocks T <= 8, the
+// remaining T running one launch per (R,T) with the chain form
+static std::vector<DepthBlock> fused_blocks(const Contract& k, int prec, int mwlt) {
+  std::vector<DepthBlock> b;
+  if (k.kernel == 3) return depth_blocks(prec, mwlt, k.depth);
+  if (k.kernel == 5 && prec == WLG_MIXED)
+    for (const DepthBlock& x : depth_blocks(prec, mwlt, k.depth))
+      if (!x.fp64) b.push_back(x);
+  return b;
+}
+
 static Contract contract_choice(const wlg_ctx* c, int prec) {
   const bool mixed = prec == WLG_MIXED;
   const int* t = c->tune;
@@ -394,11 +406,12 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int td = t[mixed ? WLG_TUNE_MIXED_CONTRACT_DEPTH : WLG_TUNE_FP64_CONTRACT_DEPTH];
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
-  // defaults (measured at 48^3x96): two-lane chain form for FP64, one-lane
-  // chain form for the FP32 transporters of the mixed path
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 2 : 4);
+  // defaults (measured at 48^3x96): two-lane chain form for FP64; on the
+  // mixed path the fused form for the FP32 blocks T <= 8 and the one-lane
+  // chain form beyond
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 5 : 4);
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[5] = {8, 512, 512, 128, 512};  // per form, measured at 48^3x96
+  static const int wg_default[6] = {8, 512, 512, 128, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : 8;
@@ -525,7 +538,8 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
       for (int i = 0; i < msl; i++) S[i] = out + i * lvl;
     }
     if (record(c, &e1)) return -1;
-    if (k.kernel == 3) {
+    int tdone = 0;  // T values covered by the fused blocks
+    if (!blocks.empty()) {
       for (size_t q = 0; q < blocks.size(); q++) {
         const DepthBlock& b = blocks[q];
         double* part = c->partial + (size_t)(b.Ta - 1) * pstride;
@@ -547,16 +561,18 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
                                                 (long)pstride, c->st);
         }
         CKL();
+        tdone = b.Ta + b.d - 1 > tdone ? b.Ta + b.d - 1 : tdone;
       }
       c->contract_launches += (long)blocks.size();
-    } else {
-      for (int t = 0; t < p->mwlt; t++) {
-        launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride,
-                                k.kernel, k.wg, c->st);
-        CKL();
-      }
-      c->contract_launches += p->mwlt;
     }
+    // form 5 continues with the one-lane chain (mixed) / two-lane chain (FP64)
+    const int kc = k.kernel == 5 ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
+    for (int t = tdone; t < p->mwlt; t++) {
+      launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride, kc,
+                              k.wg, c->st);
+      CKL();
+    }
+    c->contract_launches += p->mwlt - tdone;
     // W of this R (normalisation and wl_idx store, wloop.c:535, :584-597)
     launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, r, contract_slots(L, k.kernel, k.wg),
                         c->partial, norm, c->wl_dev, c->st);
@@ -573,8 +589,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   const int msl = p->msl;
   c->cur_msl = p->msl;
   const Contract k = contract_choice(c, prec);
-  std::vector<DepthBlock> blocks;
-  if (k.kernel == 3) blocks = depth_blocks(prec, p->mwlt, k.depth);
+  const std::vector<DepthBlock> blocks = fused_blocks(k, prec, p->mwlt);
   int nb64 = 0;
   for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
   Layout o = wloop_layout(L, p, prec, nb64);
@@ -825,10 +840,11 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 5)
+      value > 6)
     return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
-                   "per pair), 3 (chain, one lane per pair), 4 (fused time slices) or 5 (chain, "
-                   "two lanes per pair)");
+                   "per pair), 3 (chain, one lane per pair), 4 (fused time slices), 5 (chain, "
+                   "two lanes per pair) or 6 (fused time slices for the FP32 T blocks of the "
+                   "mixed path, chain form beyond)");
   if ((param == WLG_TUNE_FP64_CONTRACT_DEPTH || param == WLG_TUNE_MIXED_CONTRACT_DEPTH) &&
       value > 16)
     return set_err("time-slice depth must be 0 (default) or 1..16");

@@ -47,6 +47,19 @@ struct LevelPtrs {
   const v2<TS>* p[5];
 };
 
+// level pointer j for a per-lane j, selected from the kernel-argument
+// registers (indexing the argument block with a per-lane index makes the
+// compiler load the pointer from memory before every use)
+template <class TS>
+__device__ __forceinline__ const v2<TS>* level_ptr(const LevelPtrs<TS>& S, int j) {
+  const v2<TS>* r = S.p[0];
+  r = j == 1 ? S.p[1] : r;
+  r = j == 2 ? S.p[2] : r;
+  r = j == 3 ? S.p[3] : r;
+  r = j == 4 ? S.p[4] : r;
+  return r;
+}
+
 template <class TS>
 __global__ __launch_bounds__(256) void space_transporter(Lat L, LevelPtrs<TS> F,
                                                          LevelPtrs<TS> prev,
@@ -187,7 +200,7 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
       x[0] = c;
       M3<TS> si, l0, lr;
       if (live) {
-        load(si, S.p[i], L.V, k, s0);
+        load(si, level_ptr(S, i), L.V, k, s0);
         load(l0, LT, L.V, 0, s0);
         load(lr, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
         D = mul(dag_mul(convert<TC>(l0), convert<TC>(si)), convert<TC>(lr));
@@ -428,7 +441,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
   // this lane's levels i = h MH + m
   const v2<TS>* sp[MH];
 #pragma unroll
-  for (int m = 0; m < MH; m++) sp[m] = S.p[h * MH + m];
+  for (int m = 0; m < MH; m++) sp[m] = level_ptr(S, h * MH + m);
 
   for (int q = threadIdx.x; q < NACC * N0 * MM; q += 256) acc[q] = 0.0;
   __syncthreads();
@@ -575,10 +588,11 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // The W waves of a workgroup own the start slices x0g .. x0g+W-1 of the same
 // pair block.  At step n wave w needs, from time slice x0g+Ta+n+w, the sink
 // links S_j of the block and the time links at x and x+Rk; a ring of W+1
-// LDS slots holds the W slices of the step while the slice of the next step
-// is fetched into registers and stored into the free slot.  Level and time
-// links are fetched from HBM about (W+d-1)/(W d) times per (x0,T) instead of
-// once, and no global load is waited for inside a step.
+// LDS slots holds the W slices of the step; the slice the next step adds is
+// stored into the free slot at the top of the step, from registers loaded
+// during the step before.  Level and time links are fetched from HBM about
+// (W+d-1)/(W d) times per (x0,T) instead of once, and a global load has a
+// whole step to arrive before it is waited for.
 //
 // Lane layout: lane = 16 i + p (a row of 16 lanes per level).  The traces of a
 // step are summed over the 16 pairs of a row in FP64 by a butterfly of DPP
@@ -624,13 +638,14 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
   // this thread's elements of a ring slot: sink element q = (j*9 + e)*16 + pp
   // and time-link element q = (h*9 + e)*16 + pp (h = 1: the site x + R k)
   int sj_[PS], sp_[PS], uh_[PU], up_[PU];
-  long so_[PS], uo_[PU];
+  long uo_[PU];
+  const v2<TS>* sb_[PS];  // level plane of the element, at the element's pair
 #pragma unroll
   for (int m = 0; m < PS; m++) {
     const int q = threadIdx.x + m * NT;
     sj_[m] = q < NS ? q / 144 : -1;
     sp_[m] = q & 15;
-    so_[m] = (long)((q >> 4) % 9) * V;
+    sb_[m] = level_ptr(S, sj_[m] >= 0 ? sj_[m] : 0) + (long)((q >> 4) % 9) * V + sp_[m];
   }
 #pragma unroll
   for (int m = 0; m < PU; m++) {
@@ -662,7 +677,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       for (int m = 0; m < PS; m++) {
         ps[m].x = TS(0);
         ps[m].y = TS(0);
-        if (sj_[m] >= 0 && xs0 + sp_[m] < slice) ps[m] = S.p[sj_[m]][kV + so_[m] + ys + xs0 + sp_[m]];
+        if (sj_[m] >= 0 && xs0 + sp_[m] < slice) ps[m] = sb_[m][kV + ys + xs0];
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
@@ -743,9 +758,16 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       __syncthreads();
     }
 
+    // the slice of ring offset n+W (read by wave W-1 at step n+1) is fetched
+    // into registers during step n-1 and stored at the top of step n into the
+    // slot wave 0 read at step n-1, so a global load has a whole step to land
+    if (d > 1) fetch(W);
     for (int n = 0; n < d; n++) {
       const bool more = n + 1 < d;
-      if (more) fetch(n + W);  // the slice wave W-1 needs at step n+1
+      if (more) {
+        put((n + W) % (W + 1));
+        if (n + 2 < d) fetch(n + W + 1);
+      }
       const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
       const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
@@ -796,7 +818,6 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
           }
           D = mul(h, u);
         }
-        put((n + W) % (W + 1));
       }
       __syncthreads();
     }
