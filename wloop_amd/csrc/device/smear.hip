@@ -14,6 +14,19 @@
 
 namespace wl {
 
+// Site and output kind (direction, or direction pair) of this thread in a
+// stencil launch.  The launch is one-dimensional with ny kinds per block of 256
+// sites, the kind fastest, and every XCD takes a contiguous range of positions
+// (xcd_order): the outputs of a site block and of its neighbour blocks, which
+// read the same links, are computed close together in time on one L2.  Ordered
+// the other way (every site of one kind, then the next kind) the APE step
+// fetched 8x its algorithmic bytes from HBM.
+__device__ __forceinline__ site_t stencil_site(int ny, int& y) {
+  const int b = xcd_order(blockIdx.x, gridDim.x);
+  y = b % ny;
+  return (site_t)(b / ny) * blockDim.x + threadIdx.x;
+}
+
 // v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
 // ln: slot of the nu-links in A, lm: slot of the mu-links in B.
 template <class TC, class TS>
@@ -64,14 +77,15 @@ __device__ __forceinline__ int part4(int a, int b) {
 }
 
 // Level 1: V1[part{mu,eta}]_mu = P((1-a3) U_mu + a3/2 S_{mu eta}[U,U]);
-// blockIdx.y = 3*mu + (i-1), eta = (mu+i)&3.  Output slot part*4+mu.
+// kind y = 3*mu + (i-1), eta = (mu+i)&3.  Output slot part*4+mu.
 template <class TC, class TS>
 __global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restrict__ U,
                                                   v2<TS>* __restrict__ D1, double alpha,
                                                   double beta, int proj, int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(12, y);
   if (s >= L.V) return;
-  int mu = blockIdx.y / 3, eta = (mu + 1 + blockIdx.y % 3) & 3;
+  int mu = y / 3, eta = (mu + 1 + y % 3) & 3;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -87,9 +101,10 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
                                                   const v2<TS>* __restrict__ D1,
                                                   v2<TS>* __restrict__ D2, double alpha,
                                                   double beta, int proj, int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(12, y);
   if (s >= L.V) return;
-  int mu = blockIdx.y / 3, nu = (mu + 1 + blockIdx.y % 3) & 3;
+  int mu = y / 3, nu = (mu + 1 + y % 3) & 3;
   int n = part4(mu, nu);
   int x[4];
   coords(L, s, x);
@@ -104,15 +119,16 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
 
 // Level 3: V_mu = P((1-a1)U_mu + a1/6 sum_{i=1..3} S_{mu nu}[V2[part{nu,mu}]]),
 // nu = (mu+i)&3, written to out (which may be U itself: each thread reads
-// only its own U_mu(x) besides D2).  blockIdx.y = mu - mu0.
+// only its own U_mu(x) besides D2).  Kind y = mu - mu0 of ny.
 template <class TC, class TS>
 __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>* out,
-                                                  const v2<TS>* __restrict__ D2, int mu0,
+                                                  const v2<TS>* __restrict__ D2, int mu0, int ny,
                                                   double alpha, double beta, int proj,
                                                   int piter, int open_bc) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(ny, y);
   if (s >= L.V) return;
-  int mu = mu0 + blockIdx.y;
+  int mu = mu0 + y;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -141,9 +157,10 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256) void hyp3_level1(Lat L, const v2<TS>* __restrict__ F,
                                                    v2<TS>* __restrict__ D, double alpha,
                                                    double beta, int proj, int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(6, y);
   if (s >= L.V) return;
-  int mu = 1 + blockIdx.y / 2, eta = ((mu + blockIdx.y % 2) % 3) + 1;
+  int mu = 1 + y / 2, eta = ((mu + y % 2) % 3) + 1;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -160,9 +177,10 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
                                                    const v2<TS>* __restrict__ D,
                                                    v2<TS>* __restrict__ out, double alpha,
                                                    double beta, int proj, int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
-  int mu = 1 + blockIdx.y;
+  int mu = 1 + y;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -184,9 +202,10 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2)) void ape_step(Lat L, const v2<TS>* __restrict__ in,
                                                 v2<TS>* __restrict__ out, double alpha, int proj,
                                                 int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
-  int mu = 1 + blockIdx.y;
+  int mu = 1 + y;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -206,9 +225,10 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2
 template <class TC, class TS>
 __global__ __launch_bounds__(256, sizeof(TC) == 4 ? 4 : 2) void ape_staples(Lat L, const v2<TS>* __restrict__ in,
                                                       v2<TC>* __restrict__ S) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
-  int mu = 1 + blockIdx.y;
+  int mu = 1 + y;
   int x[4];
   coords(L, s, x);
   M3<TC> v;
@@ -225,9 +245,10 @@ __global__ __launch_bounds__(256, 3) void ape_project(Lat L, const v2<TS>* __res
                                                       const v2<TC>* __restrict__ S,
                                                       v2<TS>* __restrict__ out, double alpha,
                                                       int proj, int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
-  int mu = 1 + blockIdx.y;
+  int mu = 1 + y;
   M3<TC> v;
   load(v, S, L.V, mu - 1, s);
   decorate_store<TC, TS>(v, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
@@ -238,9 +259,10 @@ template <class TC, class TS>
 __global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict__ in,
                                                  v2<TS>* __restrict__ out, double alpha, int proj,
                                                  int piter) {
-  const site_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  int y;
+  const site_t s = stencil_site(4, y);
   if (s >= L.V) return;
-  int mu = blockIdx.y;
+  int mu = y;
   int x[4];
   coords(L, s, x);
   M3<TC> S;
@@ -268,7 +290,9 @@ __global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* _
 
 // ---- launchers --------------------------------------------------------------
 
-static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)((V + 255) / 256), ny, 1); }
+// stencil launches: ny output kinds per block of 256 sites, one-dimensional
+// (stencil_site)
+static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)(ny * ((V + 255) / 256)), 1, 1); }
 
 template <class TC, class TS>
 void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* D2,
@@ -281,7 +305,7 @@ void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* 
   int mu0 = 1 - (time > 0), mu1 = 1 + 3 * (space > 0);
   if (mu1 > mu0)
     hipLaunchKernelGGL((hyp_level3<TC, TS>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U, out,
-                       D2, mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
+                       D2, mu0, mu1 - mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
 }
 
 template <class TC, class TS>

@@ -301,6 +301,15 @@ __device__ __forceinline__ void project(M3<double>& u, int iter) {
   }
 }
 
+// ---- workgroup order ---------------------------------------------------------
+// Workgroups are dispatched to the 8 XCDs of the chip round robin (workgroup
+// b to XCD b % 8), each XCD with its own L2.  xcd_order maps b to a position
+// such that every XCD works through one contiguous range of positions.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int q = n / 8, r = n % 8, x = b % 8, l = b / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
+}
+
 // ---- lattice geometry ------------------------------------------------------
 // Lexicographic site s = x3 + N3*(x2 + N2*(x1 + N1*x0)); every direction wraps
 // periodically in index space (open boundaries are zero time links).

@@ -254,14 +254,6 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
   }
 }
 
-// Workgroups are dispatched to the 8 XCDs of the chip round robin (workgroup
-// b to XCD b % 8), each XCD with its own L2.  xcd_order maps b to a position
-// such that every XCD works through one contiguous range of positions.
-__device__ __forceinline__ int xcd_order(int b, int n) {
-  const int q = n / 8, r = n % 8, x = b % 8, l = b / 8;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + l;
-}
-
 // ---- contraction, one-lane chain form ----------------------------------
 // As the chain form, but one lane owns a (site, k) pair with the open loops
 // D_i of all msl levels, so every sink link is loaded by one lane only.  Per
