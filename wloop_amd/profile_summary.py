@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 output for the Wilson-loop kernels.
 
     python -m wloop_amd.profile_summary KT_DIR FETCH_DIR WRITE_DIR OUT_JSON
+    python -m wloop_amd.profile_summary --all KT_DIR FETCH_DIR WRITE_DIR
 
 KT_DIR holds a ``--kernel-trace --stats`` run (``*_kernel_stats.csv``),
 FETCH_DIR and WRITE_DIR ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs of the
@@ -9,7 +10,8 @@ per dispatch against the FP64 -> FP32 copy kernel ``d2f`` (one complex number
 per thread: 16 B read and 8 B written per grid item).  Prints markdown tables
 (kernel times; HBM traffic per launch of the contraction kernels) and writes
 OUT_JSON with the calibrated bytes per launch keyed "<form>/<precision>", the
-keys bench.py reads for the ``roofline.traffic`` field.
+keys bench.py reads for the ``roofline.traffic`` field.  With ``--all`` it prints
+the calibrated traffic and rate of every kernel of unfiltered counter runs.
 """
 import collections
 import csv
@@ -90,7 +92,37 @@ def traffic(fetch_dir, write_dir):
     return kf, kw, per_launch, "\n".join(table)
 
 
+def all_traffic(kt_dir, fetch_dir, write_dir):
+    """markdown table of calibrated HBM bytes and rates per launch for every
+    kernel of FETCH_DIR / WRITE_DIR runs made without a kernel filter"""
+    fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = _per_dispatch(write_dir, "WRITE_SIZE")
+    kf = [16.0 * g / b for n, g, b in fetch if re.search(r"\bd2f\b", n) and b > 0]
+    kw = [8.0 * g / b for n, g, b in write if re.search(r"\bd2f\b", n) and b > 0]
+    if not kf or not kw:
+        raise RuntimeError("no d2f dispatch to calibrate against")
+    kf, kw = sum(kf) / len(kf), sum(kw) / len(kw)
+    avg = {short(r["Name"], 200): float(r["AverageNs"]) / 1e6 for r in _csv(kt_dir, "_kernel_stats.csv")}
+    agg = collections.defaultdict(lambda: [[], []])
+    for idx, rows in ((0, fetch), (1, write)):
+        for n, g, b in rows:
+            agg[short(n, 200)][idx].append(b * (kf if idx == 0 else kw))
+    out = [f"Calibration against `d2f`: FETCH_SIZE x {kf:.4f}, WRITE_SIZE x {kw:.4f}.", "",
+           "| kernel | launches | fetched GB/launch | written GB/launch | avg ms | TB/s |",
+           "|---|---:|---:|---:|---:|---:|"]
+    for k, (f, w) in sorted(agg.items(), key=lambda kv: -sum(kv[1][0])):
+        fb = sum(f) / len(f) if f else 0.0
+        wb = sum(w) / len(w) if w else 0.0
+        t = avg.get(k, 0.0)
+        rate = f"{(fb + wb) / t / 1e9:.2f}" if t else "--"
+        out.append(f"| `{short(k)}` | {len(f)} | {fb / 1e9:.2f} | {wb / 1e9:.2f} | {t:.3f} | {rate} |")
+    return "\n".join(out)
+
+
 def main(argv):
+    if len(argv) == 5 and argv[1] == "--all":
+        print(all_traffic(*argv[2:]))
+        return 0
     if len(argv) != 5:
         print(__doc__, file=sys.stderr)
         return 2
