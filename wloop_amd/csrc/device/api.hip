@@ -524,21 +524,42 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
   const v2<TS>* S[5];
   const v2<TS>* prev[5];
   v2<TS>* bufs[2] = {sbuf0, sbuf1};
+  // T values covered by the fused blocks
+  int tfused = 0;
+  for (const DepthBlock& b : blocks) tfused = b.Ta + b.d - 1 > tfused ? b.Ta + b.d - 1 : tfused;
+  // form 5 continues with the one-lane chain (mixed) / two-lane chain (FP64)
+  const int kc = k.kernel == 5 ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
+  // with FP32 level links and the one-lane chain form for the per-(R,T)
+  // launches, the first of them forms the spatial transporters of its R (no
+  // separate pass over F, S_{R-1} and S_R)
+  const bool build = sizeof(TS) == 4 && kc == 2 && msl <= 4 && tfused < p->mwlt;
   for (int r = 0; r < p->mwlr; r++) {
     const int R = r + 1;
     hipEvent_t e0, e1, e2;
     if (record(c, &e0)) return -1;
+    const bool build_r = build && R > 1;
     if (R == 1) {
       for (int i = 0; i < msl; i++) S[i] = lev[i];
     } else {
       v2<TS>* out = bufs[R % 2];
       for (int i = 0; i < msl; i++) prev[i] = (R == 2) ? lev[i] : bufs[(R - 1) % 2] + i * lvl;
-      launch_space_transporter<TS>(L, msl, lev, prev, out, c->st);
-      CKL();
+      if (!build_r) {
+        launch_space_transporter<TS>(L, msl, lev, prev, out, c->st);
+        CKL();
+      }
       for (int i = 0; i < msl; i++) S[i] = out + i * lvl;
     }
     if (record(c, &e1)) return -1;
-    int tdone = 0;  // T values covered by the fused blocks
+    int tnext = tfused;  // first T of the per-(R,T) launches
+    if (build_r) {
+      if constexpr (sizeof(TS) == 4) {
+        launch_contract_build<TS, TC>(L, msl, R, tnext + 1, LT[tnext], lev, prev, bufs[R % 2],
+                                      c->partial + (size_t)tnext * pstride, k.wg, c->st);
+        CKL();
+      }
+      c->contract_launches += 1;
+      tnext++;
+    }
     if (!blocks.empty()) {
       for (size_t q = 0; q < blocks.size(); q++) {
         const DepthBlock& b = blocks[q];
@@ -561,18 +582,15 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
                                                 (long)pstride, c->st);
         }
         CKL();
-        tdone = b.Ta + b.d - 1 > tdone ? b.Ta + b.d - 1 : tdone;
       }
       c->contract_launches += (long)blocks.size();
     }
-    // form 5 continues with the one-lane chain (mixed) / two-lane chain (FP64)
-    const int kc = k.kernel == 5 ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
-    for (int t = tdone; t < p->mwlt; t++) {
+    for (int t = tnext; t < p->mwlt; t++) {
       launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride, kc,
                               k.wg, c->st);
       CKL();
     }
-    c->contract_launches += p->mwlt - tdone;
+    c->contract_launches += p->mwlt - tnext;
     // W of this R (normalisation and wl_idx store, wloop.c:535, :584-597)
     launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, r, contract_slots(L, k.kernel, k.wg),
                         c->partial, norm, c->wl_dev, c->st);

@@ -52,6 +52,13 @@ template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                      const v2<TS>* const* S, double* partial, int kernel, int wg,
                      hipStream_t st);
+// one-lane chain form for one (R,T) that also forms the spatial transporters
+// of R (out: msl 3-slot fields, S_R = F S_{R-1}(x+k)), which it contracts;
+// msl <= 4
+template <class TS, class TC>
+void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
+                           const v2<TS>* const* F, const v2<TS>* const* prev, v2<TS>* out,
+                           double* partial, int wg, hipStream_t st);
 // fused time-slice form: T = Ta .. Ta+d-1 of one R, W start slices per
 // workgroup (one wave each), nslot workgroups per start group; partial points
 // at the block of T = Ta, pstride apart per T

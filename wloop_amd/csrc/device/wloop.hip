@@ -47,6 +47,15 @@ struct LevelPtrs {
   const v2<TS>* p[5];
 };
 
+// Links to build the spatial transporters of one R inside a contraction
+// launch: S_R(x,k) = F(x,k) S_{R-1}(x+k,k) for every level, written to out.
+template <class TS>
+struct BuildPtrs {
+  const v2<TS>* f[5];
+  const v2<TS>* prev[5];
+  v2<TS>* out[5];
+};
+
 // level pointer j for a per-lane j, selected from the kernel-argument
 // registers (indexing the argument block with a per-lane index makes the
 // compiler load the pointer from memory before every use)
@@ -265,11 +274,16 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
 // one direction k, the three directions of a block next to each other, and
 // each XCD takes a contiguous range of units: the time transporters L_T(x) and
 // L_T(x + R k) of a block are fetched into that XCD's L2 once for the three
-// directions.  Requires msl <= 4.
-template <int MSL, class TS, class TC>
-__global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int g,
+// directions.  With BUILD the launch also forms the spatial transporters of its
+// R: every level link of a slice is computed as it is loaded, S_R(x,k) =
+// F(x,k) S_{R-1}(x+k,k) (the product and operand order of space_transporter),
+// and stored for the other launches of the R; the chains visit every slice, so
+// every S_R is formed once (the start slice of a chain twice, by the same
+// lane).  Requires msl <= 4.
+template <int MSL, class TS, class TC, bool BUILD>
+__global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, int g,
                                                        const v2<TS>* __restrict__ LT,
-                                                       LevelPtrs<TS> S,
+                                                       LevelPtrs<TS> S, BuildPtrs<TS> B,
                                                        double* __restrict__ partial) {
   constexpr int MM = MSL * MSL;
   static_assert(MM <= 16, "one-lane chain form needs msl <= 4");
@@ -298,6 +312,18 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
     const long xs = live ? xs0 : 0;
     int x[4];
     coords(L, xs, x);
+    // level link i of the pair at site s (BUILD: formed and stored as S_R)
+    auto level = [&](M3<TS>& t, int i, long s) {
+      if constexpr (BUILD) {
+        M3<TS> f, q;
+        load(f, B.f[i], L.V, k, s);
+        load(q, B.prev[i], L.V, k, fwd(L, s, x, k + 1, 1));
+        t = mul(f, q);
+        if (live) store(t, B.out[i], L.V, k, s);
+      } else {
+        load(t, S.p[i], L.V, k, s);
+      }
+    };
     M3<TC> D[MSL], Sx[MSL];
     {
       const long s0 = (long)c * slice + xs;
@@ -307,7 +333,7 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
       M3<TC> ac = convert<TC>(a), bc = convert<TC>(b);
 #pragma unroll
       for (int i = 0; i < MSL; i++) {
-        load(t, S.p[i], L.V, k, s0);
+        level(t, i, s0);
         D[i] = mul(dag_mul(ac, convert<TC>(t)), bc);
       }
     }
@@ -317,7 +343,7 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
 #pragma unroll
     for (int j = 0; j < MSL; j++) {
       M3<TS> t;
-      load(t, S.p[j], L.V, k, (long)x1 * slice + xs);
+      level(t, j, (long)x1 * slice + xs);
       Sx[j] = convert<TC>(t);
     }
     for (int n = 0; n < len; n++) {
@@ -362,7 +388,7 @@ __global__ __launch_bounds__(256) void contract_chain1(Lat L, int R, int T, int 
 #pragma unroll
         for (int j = 0; j < MSL; j++) {
           M3<TS> t;
-          load(t, S.p[j], L.V, k, (long)x1 * slice + xs);
+          level(t, j, (long)x1 * slice + xs);
           Sx[j] = convert<TC>(t);
         }
       }
@@ -907,11 +933,42 @@ static void stream_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
 
 template <int MSL, class TS, class TC>
 static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
-                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st) {
+                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st,
+                          const BuildPtrs<TS>* build = nullptr) {
   size_t lds = (size_t)L.n[0] * MSL * MSL * sizeof(double) + 2 * 4 * 16 * sizeof(double) +
                4 * 64 * 16 * sizeof(TC);
-  hipLaunchKernelGGL((contract_chain1<MSL, TS, TC>), dim3(nwg), dim3(256), lds, st, L, R, T, g, LT,
-                     sp, partial);
+  if (build)
+    hipLaunchKernelGGL((contract_chain1<MSL, TS, TC, true>), dim3(nwg), dim3(256), lds, st, L, R, T, g,
+                       LT, sp, *build, partial);
+  else
+    hipLaunchKernelGGL((contract_chain1<MSL, TS, TC, false>), dim3(nwg), dim3(256), lds, st, L, R, T, g,
+                       LT, sp, BuildPtrs<TS>{}, partial);
+}
+
+template <class TS, class TC>
+void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
+                           const v2<TS>* const* F, const v2<TS>* const* prev, v2<TS>* out,
+                           double* partial, int wg, hipStream_t st) {
+  LevelPtrs<TS> sp;
+  BuildPtrs<TS> b;
+  for (int i = 0; i < 5; i++) {
+    sp.p[i] = i < msl ? out + (size_t)i * 27 * L.V : nullptr;
+    b.f[i] = i < msl ? F[i] : nullptr;
+    b.prev[i] = i < msl ? prev[i] : nullptr;
+    b.out[i] = i < msl ? out + (size_t)i * 27 * L.V : nullptr;
+  }
+  int g = L.n[0], t = T % L.n[0];
+  while (t) {  // g = gcd(T, N0)
+    int r = g % t;
+    g = t;
+    t = r;
+  }
+  switch (msl) {
+    case 1: chain1_launch<1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
+    case 2: chain1_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
+    case 3: chain1_launch<3, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
+    default: chain1_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
+  }
 }
 
 template <int MSL, class TS, class TC>
@@ -1035,6 +1092,9 @@ template void launch_contract<double, double>(const Lat&, int, int, int, const d
                                               hipStream_t);
 template void launch_contract<float, float>(const Lat&, int, int, int, const float2*,
                                             const float2* const*, double*, int, int, hipStream_t);
+template void launch_contract_build<float, float>(const Lat&, int, int, int, const float2*,
+                                                  const float2* const*, const float2* const*, float2*,
+                                                  double*, int, hipStream_t);
 template void launch_contract_depth<double, double>(const Lat&, int, int, int, int, int, int,
                                                     const double2*, const double2*,
                                                     const double2* const*, double*, long,
