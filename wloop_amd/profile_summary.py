@@ -34,11 +34,14 @@ def _csv(d, suffix):
 
 
 def kernel_key(name):
-    """'<form>/<precision>' of a contraction kernel symbol, else None"""
+    """'<form>/<precision>' of a contraction kernel symbol, else None; the
+    one-lane chain launch that also forms the spatial transporters is form '3b'"""
     for stem, form in FORMS:
         if re.search(r"\b" + stem + r"<", name):
-            args = name[name.index("<") + 1:name.index(">")]
-            prec = "fp64" if args.split(",")[-2].strip() == "double" else "mixed"
+            args = [a.strip() for a in name[name.index("<") + 1:name.index(">")].split(",")]
+            prec = "fp64" if args[-2] == "double" else "mixed"
+            if stem == "contract_chain1" and args[-1] == "true":
+                form = "3b"
             return f"{form}/{prec}"
     return None
 
