@@ -108,11 +108,11 @@ def test_hyp_spatial_step(wl):
 C1 = dict(dims=(8, 8, 8, 8), bc=3, seed=1, eps=0.4, nss=[0, 4, 4], R=4, T=4)
 
 
-def run_both(wl, dims, bc, seed, eps, nss, R, T, precision, op=0, alpha_hyp=(0.0, 0.0)):
+def run_both(wl, dims, bc, seed, eps, nss, R, T, precision, op=0, alpha_hyp=(0.0, 0.0), proj_iter=4):
     u = O.generate(dims, bc, seed, eps)
-    po = O.make_parms(nss, T, R, alpha_ape=0.5, proj_iter=4, op_smear=op, alpha_hyp=alpha_hyp)
+    po = O.make_parms(nss, T, R, alpha_ape=0.5, proj_iter=proj_iter, op_smear=op, alpha_hyp=alpha_hyp)
     ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), po)
-    pg = wl.make_parms(nss, T, R, alpha_ape=0.5, proj_iter=4, op_smear=op, alpha_hyp=alpha_hyp)
+    pg = wl.make_parms(nss, T, R, alpha_ape=0.5, proj_iter=proj_iter, op_smear=op, alpha_hyp=alpha_hyp)
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
         got = lat.measure(pg, precision)
@@ -168,3 +168,31 @@ def test_levels_match_oracle(wl):
         lat.measure(pg)
         for sl in range(3):
             assert max_abs(lat.level(sl), ref[sl]) < 1e-13
+
+
+# level counts at the ends of the reference's range (1 <= msl <= 5,
+# wloop_parms.c:156-157; msl = 5 runs the streaming contraction, the other
+# forms need msl <= 4)
+@pytest.mark.parametrize("nss", [[0], [0, 1, 1, 1, 1]])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_measure_level_count_edges(wl, nss, precision):
+    say("msl", len(nss), "precision", precision)
+    got, ref = run_both(wl, (10, 6, 6, 6), 0, 5, 0.3, nss, 3, 5, precision)
+    assert rel_err(got, ref) < (TOL64 if precision == 0 else TOL_MIXED)
+
+
+# without projection (proj_iter = 0: proj = 0, wloop_parms.c:171)
+@pytest.mark.parametrize("precision", [0, 1])
+def test_measure_without_projection(wl, precision):
+    got, ref = run_both(wl, (8, 6, 6, 6), 3, 6, 0.3, [0, 1, 2], 3, 4, precision, proj_iter=0)
+    assert rel_err(got, ref) < (TOL64 if precision == 0 else TOL_MIXED)
+
+
+# mixed path around the FP32/FP64 split: T up to exactly 9 (the chain launch
+# that forms S_R is the only chain launch of an R), and R = 1 only (no S_R is
+# formed) with T beyond 9
+@pytest.mark.parametrize("mwlt,mwlr", [(9, 3), (10, 1)])
+def test_measure_mixed_split_edges(wl, mwlt, mwlr):
+    say("mixed split edges", mwlt, mwlr)
+    got, ref = run_both(wl, (20, 6, 6, 6), 0, 7, 0.3, [0, 2, 3], mwlr, mwlt, wl.MIXED)
+    assert rel_err(got, ref) < TOL_MIXED
