@@ -73,12 +73,13 @@ def time_path(lat, parms, precision, steps, warmup):
 
 def cpu_baseline(wl_cfg, parms_o):
     """Oracle (the reference loop structure, FP64) timed on this host's cores on a
-    48^3 x 16 slab of the same workload (one HYP, one APE step, one L_T and one
-    S_R update, one (R,T) contraction: about 10-15 s of 16-core work),
-    extrapolated per site and per stage count."""
+    48^3 x 32 slab of the same workload (one HYP, one APE step, one L_T and one
+    S_R update, one (R,T) contraction: about 10 s of 16-core work; the oracle
+    spreads its threads over time slices, so the slab has at least as many
+    slices as threads), extrapolated per site and per stage count."""
     import oracle as O
     dims = wl_cfg["dims"]
-    sdims = (min(16, dims[0]),) + tuple(dims[1:])
+    sdims = (min(32, dims[0]),) + tuple(dims[1:])
     u = O.generate(sdims, 3, wl_cfg["seed"], wl_cfg["eps"])
     cores = min(16, len(os.sched_getaffinity(0)))
     t = O.time_stages(sdims, 3, u, ALPHA_ACTION, parms_o, cores)
