@@ -300,6 +300,21 @@ def main():
                    "max_rel_err_vs_fp64": errs[0] if errs else None}
         log(f"[bench] batched mixed: {batched}")
 
+    # drop-in boundary: there the gauge field arrives in host memory (openQCD's
+    # udfld(), pageable): upload = host -> device copy + AoS -> SoA on the device
+    u_host = lat.download()
+    lat.upload(u_host)
+    lat.synchronize()
+    t0 = time.perf_counter()
+    lat.upload(u_host)
+    lat.synchronize()
+    up_ms = (time.perf_counter() - t0) * 1e3
+    del u_host
+    host_input = {"upload_ms": up_ms, "bytes": int(np.prod(dims)) * 4 * 144,
+                  "mixed_ms_per_config_incl_upload": tmx / args.steps * 1e3 + up_ms,
+                  "mixed_configs_per_s_incl_upload": 1e3 / (tmx / args.steps * 1e3 + up_ms)}
+    log(f"[bench] host input: {host_input}")
+
     V = int(np.prod(dims))
     links_per_site = 1 + 3 * parms.msl            # L_T + S_R of every level and direction
 
@@ -388,6 +403,7 @@ def main():
         "autotuned_contraction": tuned,
         "batched_mixed": batched,
         "mixed_max_rel_err_vs_fp64": err,
+        "host_input": host_input,
         "stage_ms_per_config": {k: v / args.steps for k, v in timmx.items()},
         "roofline": roof_mx,
     }
