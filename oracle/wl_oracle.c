@@ -1,7 +1,7 @@
 /*
  * wl_oracle.c -- CPU FP64 restatement of the wloop measurement path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): loaded by
+ * TEST INFRASTRUCTURE ONLY (the reference of the tests, never the product): loaded by
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
  *
  * The reference (bjoern-leder/wloop, wloop-1.4) cannot be built here: it

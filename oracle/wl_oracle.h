@@ -1,9 +1,9 @@
 /*
  * wl_oracle.h -- CPU FP64 restatement of the wloop measurement path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
- * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
- * may load it.  The product library (libwloop_amd.so) never links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This library is the FP64 reference the HIP path
+ * is tested against: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library (libwloop_amd.so) never links or calls it.
  *
  * Field layout used throughout the oracle: lexicographic sites with x3
  * fastest, s = x3 + N3*(x2 + N2*(x1 + N1*x0)); a link field is

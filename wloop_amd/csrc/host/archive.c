@@ -1,5 +1,5 @@
 /*
- * archive.c -- gauge configurations for the driver: openQCD's exported file
+ * archive.c -- gauge configurations for the ms10 program: openQCD's exported file
  * format (import_cnfg / export_cnfg, as called at main/ms10.c:712-717) and a
  * seeded synthetic generator on the GPU.
  *
