@@ -84,8 +84,12 @@ int wlg_wloop(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
 int wlg_measure(wlg_ctx *ctx, const wlg_parms *p, int precision, double *wl);
 /* operator levels only (smear.c:99-121) of the resident field */
 int wlg_levels(wlg_ctx *ctx, const wlg_parms *p, int precision);
-/* spatial links of operator level sl of the last wlg_levels/wlg_wloop/wlg_measure,
- * host AoS [V][3][18] */
+/* spatial links of operator level sl (0 <= sl < msl) of the last
+ * wlg_levels/wlg_wloop/wlg_measure, or for msl <= sl < 2 msl the spatial
+ * transporters S_R of level sl-msl for the last R (>= 2) of the last
+ * wlg_wloop/wlg_measure (the reference's pus[msl+sl] after wloop_sum); host
+ * AoS [V][3][18].  Fails once a later operation on the context (upload,
+ * generation, smearing, plaquette, staple sum, trim) has reused their memory. */
 int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
 
 /* Batches (no reference counterpart): the full measurement (wlg_measure) of
@@ -93,8 +97,10 @@ int wlg_level_download(wlg_ctx *ctx, int sl, double *lev);
  * HIP graph whose per-context branches run concurrently.  wlg_batch_run
  * replays it on the fields currently resident in the contexts (refresh them
  * with wlg_generate/wlg_upload between runs) and copies W of context i to
- * wl[i].  Buffers are allocated at creation; a context must not be measured
- * eagerly with other parameters while it belongs to a batch. */
+ * wl[i].  Buffers are allocated at creation.  While a context belongs to a
+ * batch its buffers are never reallocated or freed: an eager call that needs
+ * more memory, wlg_trim and a change of wlg_tune fail with an error instead
+ * (destroy the batch first).  Destroy a batch before its contexts. */
 typedef struct wlg_batch wlg_batch;
 int wlg_batch_create(wlg_batch **batch, int n, wlg_ctx *const *ctxs, const wlg_parms *parms,
                      int precision);

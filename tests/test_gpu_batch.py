@@ -44,3 +44,28 @@ def test_batch_rejects_duplicate_context(wl):
     with pytest.raises(wl.WloopError):
         wl.Batch([lat, lat], [p, p])
     lat.close()
+
+
+def test_batched_context_buffers_are_held(wl):
+    """While a batch graph uses a context's buffers, nothing may free or move
+    them: trim, a tuning change and an eager measurement that needs more memory
+    fail; after the batch is destroyed they work again."""
+    lat = wl.Lattice((8, 4, 4, 4), 3)
+    p = wl.make_parms([0, 1], 2, 2)
+    batch = wl.Batch([lat], [p], wl.FP64)
+    lat.generate(5, 0.3)
+    first = batch.run()[0]
+    with pytest.raises(wl.WloopError, match="batch"):
+        lat.trim()
+    with pytest.raises(wl.WloopError, match="batch"):
+        lat.tune(wl.TUNE_FP64_CONTRACT_WG, 2048)
+    big = wl.make_parms([0, 1, 2, 3, 4], 5, 3)
+    with pytest.raises(wl.WloopError, match="batch"):
+        lat.measure(big, wl.FP64)
+    assert np.array_equal(lat.measure(p, wl.FP64), first)   # fits: allowed
+    assert np.array_equal(batch.run()[0], first)
+    batch.close()
+    lat.trim()
+    lat.tune(wl.TUNE_FP64_CONTRACT_WG, 2048)
+    lat.measure(big, wl.FP64)
+    lat.close()

@@ -162,3 +162,72 @@ def test_lnk_plaq_sum(C, bc):
                 tr = tr[:-1]
             ref += tr.sum()
     assert abs(got - ref) < 1e-10 * abs(ref)
+
+
+def long_links(levels, dims, R):
+    """S_R^i(x,k) = F^i_k(x) S_{R-1}^i(x+k,k) with S_1 = F (wloop.c:207-258),
+    in numpy from the oracle's levels ([V][3][18] each)."""
+    out = []
+    for lev in levels:
+        f = O.to_complex(lev.reshape(-1, 1, 18)).reshape(dims + (3, 3, 3))
+        s = f.copy()
+        for _ in range(R - 1):
+            s = np.stack([f[..., k, :, :] @ np.roll(s[..., k, :, :], -1, axis=k + 1)
+                          for k in range(3)], axis=-3)
+        out.append(s)
+    return out
+
+
+def pus_level(C, pus, ipt, V, sl):
+    return np.array([[np.ctypeslib.as_array(pus[k + 3 * (int(ipt[l]) + V * sl)], shape=(18,))
+                      for k in range(3)] for l in range(V)])
+
+
+@pytest.mark.parametrize("between", ["plaq", "hyp"])
+def test_get_pus_after_other_gpu_work(C, between):
+    """Levels of smear() stay those of smear(): a later plaquette or HYP on
+    udfld() reuses the device memory they lived in (smear.c:99-127)."""
+    dims = (4, 6, 6, 6)
+    ud, ipt, offs = setup_lattice(C, dims, 3)
+    u = O.generate(dims, 3, 21, 0.4)
+    write_field(ud, offs, u)
+    C.set_flags(1)
+    set_parms(C, [1, 2], 2, 2)
+    C.smear()
+    if between == "plaq":
+        C.lnk_plaq_sum(C.udfld())
+    else:
+        C.hyp_links(1.0, 1.0, 0.5, 1, 4, 0)
+    pus = C.get_pus()
+    ref = O.levels(dims, u, O.make_parms([1, 2], 2, 2))
+    V = int(np.prod(dims))
+    for sl in range(2):
+        assert np.max(np.abs(pus_level(C, pus, ipt, V, sl) - ref[sl])) < 1e-13
+    C.free_smear()
+
+
+@pytest.mark.parametrize("keep", [0, 1])
+def test_get_pus_after_wloop_sum(C, keep):
+    """After wloop_sum the table holds the levels and, in the upper msl slots,
+    the spatial transporters of the last R (wloop.c:207-258, :524-608); free_wloop
+    leaves them in place (wloop.c:511-521)."""
+    dims = (4, 6, 6, 6)
+    nss, T, R = [0, 2], 2, 3
+    ud, ipt, offs = setup_lattice(C, dims, 3)
+    u = O.generate(dims, 3, 22, 0.35)
+    write_field(ud, offs, u)
+    C.set_flags(1)
+    set_parms(C, nss, T, R)
+    C.wloop_sum(keep)
+    if keep:
+        C.free_wloop()
+    pus = C.get_pus()
+    V = int(np.prod(dims))
+    ref = O.levels(dims, u, O.make_parms(nss, T, R))
+    for sl in range(2):
+        assert np.max(np.abs(pus_level(C, pus, ipt, V, sl) - ref[sl])) < 1e-13
+    sr = long_links(ref, dims, R)
+    for sl in range(2):
+        got = O.to_complex(pus_level(C, pus, ipt, V, 2 + sl).reshape(-1, 1, 18)).reshape(sr[sl].shape)
+        assert np.max(np.abs(got - sr[sl])) < 1e-13
+    C.free_smear()

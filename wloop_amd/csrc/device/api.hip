@@ -59,7 +59,12 @@ struct wlg_ctx {
   // operator levels of the last measurement (FP64, 3-slot fields)
   const double2* lev[5];
   const float2* lev32[5];
-  int msl, lev_fp32;
+  int msl, lev_fp32;   // msl = 0: no valid levels
+  // spatial transporters S_R of the last R >= 2 of the last measurement (the
+  // reference's pus[msl+sl] after wloop_sum), valid while slong_ok
+  const void* slong[5];
+  int slong_ok;
+  int batch_refs;      // batches whose captured graph uses this context's buffers
   // timing
   std::vector<hipEvent_t> ev;
   size_t ev_used;
@@ -77,18 +82,53 @@ struct wlg_ctx {
 
 extern "C" const char* wlg_error(void) { return g_err; }
 
+int wl_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return -1;
+}
+
+int wl_resident_field(wlg_ctx* c, int* dev, const Lat** L, const double2** U, hipStream_t* st,
+                      int* open_bc) {
+  if (!c) return set_err("no context");
+  *dev = c->device;
+  *L = &c->L;
+  *U = c->U;
+  *st = c->st;
+  *open_bc = c->bc == 0;
+  return 0;
+}
+
 static size_t slot64(const Lat& L) { return (size_t)9 * L.V * sizeof(double2); }
 static size_t slot32(const Lat& L) { return (size_t)9 * L.V * sizeof(float2); }
 
-static int ensure(char** p, size_t* have, size_t need) {
+// Grow a device buffer of context c to at least `need` bytes.  While a batch
+// graph holds the context (its kernels were captured with the current buffer
+// addresses) a buffer is never reallocated: the call fails instead.
+static int ensure(wlg_ctx* c, char** p, size_t* have, size_t need) {
   if (*have >= need) return 0;
   if (g_capturing) return set_err("device buffer of %zu bytes not prepared before graph capture", need);
+  if (c->batch_refs > 0)
+    return set_err("the context belongs to %d batch(es) whose graph uses its buffers: destroy the "
+                   "batch before measuring it with parameters that need %zu bytes more",
+                   c->batch_refs, need - *have);
   if (*p) CK(hipFree(*p));
   *p = nullptr;
   *have = 0;
   CK(hipMalloc((void**)p, need));
   *have = need;
   return 0;
+}
+
+// The scratch arena of c, grown to `need` bytes, for an operation that
+// overwrites it: the operator levels and spatial transporters of the last
+// measurement live there, so they are no longer available afterwards.
+static int scratch(wlg_ctx* c, size_t need) {
+  c->msl = 0;
+  c->slong_ok = 0;
+  return ensure(c, &c->arena, &c->arena_bytes, need);
 }
 
 static hipEvent_t next_event(wlg_ctx* c) {
@@ -171,7 +211,7 @@ extern "C" int wlg_synchronize(wlg_ctx* c) {
 // host AoS [V][nl][18] <-> device planes, staged through the arena
 static int upload_field(wlg_ctx* c, const double* h, int nl, double2* dst) {
   size_t bytes = (size_t)c->L.V * nl * 18 * sizeof(double);
-  if (ensure(&c->arena, &c->arena_bytes, bytes)) return -1;
+  if (scratch(c, bytes)) return -1;
   CK(hipMemcpyAsync(c->arena, h, bytes, hipMemcpyHostToDevice, c->st));
   launch_aos_to_soa(c->L.V, nl, (const double*)c->arena, dst, c->st);
   CKL();
@@ -180,7 +220,7 @@ static int upload_field(wlg_ctx* c, const double* h, int nl, double2* dst) {
 
 static int download_field(wlg_ctx* c, const double2* src, int nl, double* h) {
   size_t bytes = (size_t)c->L.V * nl * 18 * sizeof(double);
-  if (ensure(&c->arena, &c->arena_bytes, bytes)) return -1;
+  if (scratch(c, bytes)) return -1;
   launch_soa_to_aos(c->L.V, nl, src, (double*)c->arena, c->st);
   CKL();
   CK(hipMemcpyAsync(h, c->arena, bytes, hipMemcpyDeviceToHost, c->st));
@@ -202,7 +242,7 @@ extern "C" int wlg_download(wlg_ctx* c, double* u) {
 
 extern "C" int wlg_generate(wlg_ctx* c, uint64_t seed, double eps) {
   CK(hipSetDevice(c->device));
-  if (ensure(&c->arena, &c->arena_bytes, slot64(c->L))) return -1;
+  if (scratch(c, slot64(c->L))) return -1;
   launch_generate(c->L, c->U, (double2*)c->arena, seed, eps, c->bc == 0, c->st);
   CKL();
   CK(hipStreamSynchronize(c->st));
@@ -218,7 +258,7 @@ static int run_hyp(wlg_ctx* c, const double2* in, double2* out, const double a[3
     CKL();
   }
   if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
-  if (ensure(&c->arena, &c->arena_bytes, 24 * s)) return -1;
+  if (scratch(c, 24 * s)) return -1;
   double2* d1 = (double2*)c->arena;
   double2* d2 = (double2*)(c->arena + 12 * s);
   if (prec == WLG_MIXED)
@@ -247,7 +287,7 @@ extern "C" int wlg_ape_step(wlg_ctx* c, double alpha, int proj, int proj_iter, i
                             int prec) {
   CK(hipSetDevice(c->device));
   size_t s = slot64(c->L);
-  if (ensure(&c->arena, &c->arena_bytes, 4 * s)) return -1;
+  if (scratch(c, 4 * s)) return -1;
   double2* tmp = (double2*)c->arena;
   if (four_d) {
     launch_ape4(c->L, c->U, tmp, alpha, proj, proj_iter, c->st);
@@ -270,7 +310,7 @@ extern "C" int wlg_hyp_spatial_step(wlg_ctx* c, double a1, double a2, int proj, 
                                     int prec) {
   CK(hipSetDevice(c->device));
   size_t s = slot64(c->L);
-  if (ensure(&c->arena, &c->arena_bytes, 9 * s)) return -1;
+  if (scratch(c, 9 * s)) return -1;
   double2* D = (double2*)c->arena;
   double2* out = (double2*)(c->arena + 6 * s);
   const double2* F = c->U + 9 * c->L.V;
@@ -291,7 +331,7 @@ extern "C" int wlg_staple_sum(wlg_ctx* c, const double* a, const double* b, int 
     return set_err("staple_sum: need 0 <= mu,nu <= 3 and mu != nu (got %d,%d)", mu, nu);
   size_t s = slot64(c->L);
   size_t aos = (size_t)c->L.V * 4 * 18 * sizeof(double);
-  if (ensure(&c->arena, &c->arena_bytes, 12 * s + aos)) return -1;
+  if (scratch(c, 12 * s + aos)) return -1;
   double2* A = (double2*)c->arena;
   double2* B = (double2*)(c->arena + 4 * s);
   double2* Vv = (double2*)(c->arena + 8 * s);
@@ -441,7 +481,10 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  const Contract k = {0, 32, 16, 8};  // the most partial sums among the tuning candidates
+  // partial sums per x0: the most among the library defaults and the tuning
+  // candidates (streaming form: 3 x 32 chunks; chain forms: up to 2048
+  // persistent workgroups)
+  const Contract k = {1, 2048, 16, 8};
   return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, k) * sizeof(double);
 }
 
@@ -597,6 +640,12 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     CKL();
     if (record(c, &e2)) return -1;
   }
+  // the transporters of the last R stay in the arena (pus[msl+sl] of the
+  // reference after wloop_sum, wloop.c:207-258)
+  if (p->mwlr >= 2) {
+    for (int i = 0; i < msl; i++) c->slong[i] = S[i];
+    c->slong_ok = 1;
+  }
   return 0;
 }
 
@@ -611,10 +660,10 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   int nb64 = 0;
   for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
   Layout o = wloop_layout(L, p, prec, nb64);
-  if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
+  if (scratch(c, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
-  if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
+  if (ensure(c, (char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
+  if (ensure(c, (char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
     return -1;
   const size_t s64 = slot64(L), s32 = slot32(L);
   char* A = c->arena;
@@ -760,17 +809,24 @@ extern "C" int wlg_measure(wlg_ctx* c, const wlg_parms* p, int prec, double* wl)
 
 extern "C" int wlg_level_download(wlg_ctx* c, int sl, double* lev) {
   CK(hipSetDevice(c->device));
-  if (sl < 0 || sl >= c->msl) return set_err("level %d not available (msl=%d)", sl, c->msl);
+  if (c->msl == 0)
+    return set_err("no operator levels on the device: none were built, or a later operation on "
+                   "the context reused their memory");
+  if (sl < 0 || sl >= 2 * c->msl) return set_err("level %d not available (msl=%d)", sl, c->msl);
+  if (sl >= c->msl && !c->slong_ok)
+    return set_err("no spatial transporters of R >= 2 from the last measurement (slot %d)", sl);
   size_t bytes = (size_t)c->L.V * 3 * 18 * sizeof(double);
   double* stage = nullptr;
   double2* wide = nullptr;
   CK(hipMalloc((void**)&stage, bytes));
   hipError_t e = hipSuccess;
-  const double2* src = c->lev[sl];
+  const bool up = sl >= c->msl;
+  const double2* src = up ? (const double2*)c->slong[sl - c->msl] : c->lev[sl];
   if (c->lev_fp32) {
     e = hipMalloc((void**)&wide, bytes);
     if (e == hipSuccess) {
-      launch_f2d((long)27 * c->L.V, c->lev32[sl], wide, c->st);
+      launch_f2d((long)27 * c->L.V, up ? (const float2*)c->slong[sl - c->msl] : c->lev32[sl], wide,
+                 c->st);
       e = hipGetLastError();
     }
     src = wide;
@@ -807,7 +863,7 @@ extern "C" int wlg_levels(wlg_ctx* c, const wlg_parms* p, int prec) {
   CK(hipSetDevice(c->device));
   if (check_parms(c, p)) return -1;
   Layout o = wloop_layout(c->L, p, prec, 0);
-  if (ensure(&c->arena, &c->arena_bytes, o.total)) return -1;
+  if (scratch(c, o.total)) return -1;
   const double2* lev[5];
   const float2* lev32[5];
   if (run_levels(c, c->U, p, prec, o, lev, lev32)) return -1;
@@ -827,6 +883,9 @@ extern "C" int wlg_set_bc(wlg_ctx* c) {
 
 extern "C" int wlg_trim(wlg_ctx* c) {
   CK(hipSetDevice(c->device));
+  if (c->batch_refs > 0)
+    return set_err("the context belongs to %d batch(es): destroy them before releasing its memory",
+                   c->batch_refs);
   CK(hipStreamSynchronize(c->st));
   if (c->arena) CK(hipFree(c->arena));
   if (c->partial) CK(hipFree(c->partial));
@@ -836,13 +895,14 @@ extern "C" int wlg_trim(wlg_ctx* c) {
   c->Uh = nullptr;
   c->arena_bytes = c->partial_bytes = 0;
   c->msl = 0;
+  c->slong_ok = 0;
   return 0;
 }
 
 extern "C" int wlg_plaquette(wlg_ctx* c, double* sum) {
   CK(hipSetDevice(c->device));
   int nblk = plaquette_blocks(c->L);
-  if (ensure(&c->arena, &c->arena_bytes, (size_t)nblk * sizeof(double))) return -1;
+  if (scratch(c, (size_t)nblk * sizeof(double))) return -1;
   launch_plaquette(c->L, c->U, c->bc == 0, (double*)c->arena, nblk, c->st);
   CKL();
   std::vector<double> part(nblk);
@@ -856,6 +916,9 @@ extern "C" int wlg_plaquette(wlg_ctx* c, double* sum) {
 
 extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if (param < 0 || param >= WLG_TUNE_COUNT) return set_err("unknown tuning parameter %d", param);
+  if (c->batch_refs > 0 && c->tune[param] != value)
+    return set_err("the context belongs to %d batch(es) captured with its launch shapes: destroy "
+                   "them before changing a tuning parameter", c->batch_refs);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
       value > 6)
@@ -893,10 +956,10 @@ static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
   const int nb64 = k.kernel == 3 ? depth_fp64_starts(prec, p->mwlt, k.depth) : 0;
   Layout o = wloop_layout(L, p, prec, nb64);
   size_t hyp = 24 * slot64(L);
-  if (ensure(&c->arena, &c->arena_bytes, o.total > hyp ? o.total : hyp)) return -1;
+  if (scratch(c, o.total > hyp ? o.total : hyp)) return -1;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * L.n[0];
-  if (ensure((char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
-  if (ensure((char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
+  if (ensure(c, (char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
+  if (ensure(c, (char**)&c->partial, &c->partial_bytes, partial_count(L, p, k) * sizeof(double)))
     return -1;
   if (c->wl_pinned_bytes < wls * sizeof(double)) {
     if (c->wl_pinned) CK(hipHostFree(c->wl_pinned));
@@ -1001,6 +1064,7 @@ extern "C" int wlg_batch_create(wlg_batch** out, int n, wlg_ctx* const* ctxs,
     batch_free(b);
     return -1;
   }
+  for (int i = 0; i < n; i++) ctxs[i]->batch_refs++;
   *out = b;
   return 0;
 }
@@ -1029,5 +1093,6 @@ extern "C" void wlg_batch_destroy(wlg_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->st);
+  for (wlg_ctx* c : b->ctx) c->batch_refs--;
   batch_free(b);
 }

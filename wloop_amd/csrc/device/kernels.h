@@ -72,3 +72,14 @@ void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, co
                          double norm, double* wl, hipStream_t st);
 
 }  // namespace wl
+
+// ---- internal services of api.hip for the other files of the library ----
+struct wlg_ctx;
+// record an error message for wlg_error(); returns -1
+__attribute__((visibility("hidden"))) int wl_set_error(const char* fmt, ...);
+// the resident gauge field of a context and what launching on it needs
+__attribute__((visibility("hidden"))) int wl_resident_field(wlg_ctx* c, int* dev,
+                                                            const wl::Lat** L,
+                                                            const double2** U, hipStream_t* st,
+                                                            int* open_bc);
+

@@ -6,8 +6,14 @@
 #include "su3.h"
 
 /* device context holding the current gauge field; uploads udfld() first if
- * the host copy is newer */
+ * the host copy is newer.  Every caller is about to run a device operation
+ * that may reuse the memory of the operator levels (wl_levels_guard). */
 wlg_ctx *wl_gauge_ctx(void);
+/* the same context as it is (NULL if none), no upload and no level guard */
+wlg_ctx *wl_gauge_ctx_raw(void);
+/* keep the operator levels of smear() for get_pus() before c's scratch
+ * memory is reused */
+void wl_levels_guard(wlg_ctx *c);
 /* the device copy of the gauge field has just been changed */
 void wl_device_changed(void);
 /* host storage of udfld() without coherence side effects (may be NULL) */

@@ -146,6 +146,7 @@ void set_bc_type(int type)
    error((type != 0) && (type != 3), 1, "set_bc_type [runtime.c]",
          "Only open (0) and periodic (3) boundary conditions are supported");
    if (type != bc) {
+      if (ctx) wl_levels_guard(ctx);
       if (ctx) wlg_destroy(ctx);
       if (aux) wlg_destroy(aux);
       ctx = aux = NULL;
@@ -162,6 +163,7 @@ void set_wloop_device(int dev)
    if (dev != device) {
       su3_dble *u = udfld();   /* keep the field on the host while switching */
       (void)u;
+      if (ctx) wl_levels_guard(ctx);
       if (ctx) wlg_destroy(ctx);
       if (aux) wlg_destroy(aux);
       ctx = aux = NULL;
@@ -185,6 +187,7 @@ su3_dble *wl_host_field(void) { return ud; }
 static void download_if_needed(void)
 {
    if (device_newer && ctx) {
+      wl_levels_guard(ctx);
       double *aos = malloc(sizeof(double) * 72 * (size_t)VOLUME);
       error(aos == NULL, 1, "udfld [runtime.c]", "Unable to allocate transfer buffer");
       wl_check(wlg_download(ctx, aos), "udfld [runtime.c]");
@@ -252,6 +255,8 @@ static wlg_ctx *new_ctx(void)
    return c;
 }
 
+wlg_ctx *wl_gauge_ctx_raw(void) { return ctx; }
+
 wlg_ctx *wl_gauge_ctx(void)
 {
    geometry();
@@ -260,6 +265,7 @@ wlg_ctx *wl_gauge_ctx(void)
       ctx = new_ctx();
       host_newer = 1;
    }
+   wl_levels_guard(ctx);
    if (host_newer) {
       double *aos = malloc(sizeof(double) * 72 * (size_t)VOLUME);
       error(aos == NULL, 1, "wl_gauge_ctx [runtime.c]", "Unable to allocate transfer buffer");
