@@ -1,0 +1,136 @@
+"""GPU tests at the benchmark configurations themselves.
+
+* 48^3x96 with open boundaries (the headline precision form): the mixed path
+  against the FP64 path (every W_ij(R,T;x0) within 1e-6 relative), W exactly
+  zero exactly where the loop crosses the zero time links (x0 >= N0-T,
+  wloop.c:153-204 with set_bc), and FP64 results identical run to run.
+* The same form at 32^3x64 against the oracle fixture (FP64 to 1e-12, mixed to
+  1e-6, each contraction form the library offers for it).
+* The 16-lattice batch (6x 16^3x32, 6x 24^3x48, 4x 32^3x64): one captured graph
+  equals the eager measurements bit for bit; two members against fixtures.
+* A serving stream over the serving sizes up to 32^3x64, both precisions: every
+  served W equals the eager measurement of the same configuration.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+from make_golden import c4_members  # noqa: E402
+
+AA = (1.0, 1.0, 0.5)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def as_array(w, p, n0):
+    """[x0, R-1, T-1, i, j] view of a wl_idx-ordered W"""
+    return w.reshape(n0, p.mwlr, p.mwlt, p.msl, p.msl).transpose(0, 1, 2, 4, 3)
+
+
+def zero_pattern_ok(W, n0, mwlt):
+    """open bc: W(R,T;x0) == 0 exactly iff x0 >= N0 - T"""
+    for t in range(mwlt):
+        T = t + 1
+        crossing = np.arange(n0) >= n0 - T
+        z = np.all(W[:, :, t] == 0.0, axis=(1, 2, 3))
+        nz = np.all(W[:, :, t] != 0.0, axis=(1, 2, 3))
+        if not (np.array_equal(z, crossing) and np.array_equal(nz, ~crossing)):
+            return False
+    return True
+
+
+def test_headline_48c96_open(wl):
+    dims = (96, 48, 48, 48)
+    p = wl.make_parms([0, 16, 32, 48], 16, 16, alpha_ape=0.5, proj_iter=4, alpha_action=AA)
+    with wl.Lattice(dims, wl.OPEN) as lat:
+        lat.generate(3, 0.25)
+        say("48^3x96 open: FP64 run 1")
+        w64 = lat.measure(p, wl.FP64)
+        say("48^3x96 open: FP64 run 2")
+        w64b = lat.measure(p, wl.FP64)
+        say("48^3x96 open: mixed")
+        wmx = lat.measure(p, wl.MIXED)
+    assert np.array_equal(w64, w64b), "FP64 results differ between runs"
+    err = rel_err(wmx, w64)
+    say(f"48^3x96 open: mixed vs FP64 max rel err {err:.3e}")
+    assert err < 1e-6
+    for w in (w64, wmx):
+        assert zero_pattern_ok(as_array(w, p, 96), 96, 16)
+
+
+def fixture(name):
+    return np.load(os.path.join(HERE, "golden", name))
+
+
+@pytest.mark.parametrize("precision,kernel", [(0, 0), (0, 4), (1, 0), (1, 4)])
+def test_precision_form_32c64_vs_oracle(wl, precision, kernel):
+    """kernel 0: library default (two-lane chain FP64; fused FP32 blocks then
+    one-lane chain, S_R formed in the chain launch, mixed); 4: fused time
+    slices for every T (FP64 blocks from T = 9 on the mixed path)"""
+    g = fixture("c3s_w_sub.npz")
+    dims = (64, 32, 32, 32)
+    p = wl.make_parms([0, 16, 32, 48], 16, 16, alpha_ape=0.5, proj_iter=4, alpha_action=AA)
+    say(f"32^3x64 open vs oracle: precision {precision} kernel {kernel}")
+    with wl.Lattice(dims, wl.OPEN) as lat:
+        lat.generate(3, 0.25)
+        if kernel:
+            lat.tune(wl.TUNE_MIXED_CONTRACT_KERNEL if precision else wl.TUNE_FP64_CONTRACT_KERNEL, kernel)
+        w = as_array(lat.measure(p, precision), p, 64)
+    tol = 1e-12 if precision == 0 else 1e-6
+    e = rel_err(w[list(g["x0"])], g["w"])
+    es = rel_err(w.sum(axis=0), g["wsum"])
+    say(f"  max rel err: x0 slices {e:.2e}, sum over x0 {es:.2e}")
+    assert e < tol and es < tol
+    assert zero_pattern_ok(w, 64, 16)
+
+
+def test_batch_16_lattices(wl):
+    members = c4_members()
+    lats, parms = [], []
+    for L, seed, eps in members:
+        lats.append(wl.Lattice((2 * L, L, L, L), wl.PERIODIC))
+        parms.append(wl.make_parms([0, 8, 16], L // 2, L // 2, alpha_ape=0.5, proj_iter=4, alpha_action=AA))
+    say("batch of 16 lattices: capture")
+    batch = wl.Batch(lats, parms, wl.MIXED)
+    for rep in range(2):
+        for i, (L, seed, eps) in enumerate(members):
+            lats[i].generate(seed + 1000 * rep, eps)
+        say(f"batch replay {rep}")
+        got = batch.run()
+        for i, lat in enumerate(lats):
+            assert np.array_equal(got[i], lat.measure(parms[i], wl.MIXED)), (rep, i)
+        if rep == 0:
+            for i in (0, 6):
+                L = members[i][0]
+                g = fixture(f"c4_{L}_w_sub.npz")
+                w = as_array(got[i], parms[i], 2 * L)
+                e = max(rel_err(w[list(g["x0"])], g["w"]), rel_err(w.sum(axis=0), g["wsum"]))
+                say(f"  member {i} ({L}^3x{2 * L}) vs oracle: {e:.2e}")
+                assert e < 1e-6
+    batch.close()
+    for lat in lats:
+        lat.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_serving_stream(wl, precision):
+    from wloop_amd import serve
+    sizes = (12, 16, 24, 32)
+    say(f"serving stream, precision {precision}")
+    out, reqs = serve.run_stream(50.0, 10, precision, 260.0, workers=2, sizes=sizes, seed=11)
+    say(out)
+    assert out["completed"] == 10 and out["rejected"] == 0
+    for r in reqs:
+        with wl.Lattice(r.dims, r.bc) as lat:
+            lat.generate(r.seed, r.eps)
+            assert np.array_equal(lat.measure(r.parms, r.precision), r.result), r.dims
