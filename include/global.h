@@ -35,6 +35,9 @@ extern int wl_volume;
 #define FACE3 0
 #define BNDRY 0
 
+/* the openQCD interface the library supplies (the pieces the path calls) */
+#define openQCD_RELEASE "openQCD-1.4"
+
 #define NAME_SIZE 128
 #define ALIGN 6
 

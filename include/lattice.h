@@ -55,6 +55,15 @@ void import_cnfg(const char *in);
 void export_cnfg(const char *out);
 /* lattice sizes from the header of an exported configuration */
 void cnfg_sizes(const char *in, int n[4]);
+/* Imported (per-process) format, as ms10 -noexp reads it (main/ms10.c:706-713):
+ * 16 native ints (process grid, local sizes, process block sizes, process
+ * coordinates), the local plaquette sum (double), then udfld() as it lies in
+ * memory.  One process owns the lattice here, so the grid is 1x1x1x1.
+ * read_cnfg checks the header and the plaquette sum. */
+void read_cnfg(const char *in);
+void write_cnfg(const char *out);
+/* local lattice sizes from the header of an imported configuration */
+void cnfg_sizes_imported(const char *in, int n[4]);
 /* No reference counterpart: a seeded synthetic configuration in udfld(),
  * generated on the GPU (U = exp(i eps H), Haar gauge transform, set_bc) */
 void random_cnfg(unsigned long long seed, double eps);

@@ -96,3 +96,29 @@ def test_exported_configurations(tmp_path, wl):
         u = read_exported(f"{d}/cnfg/expn{nc}", DIMS)
         assert np.max(np.abs(u - O.generate(DIMS, 0, 100 + nc, 0.3))) < 1e-13
         assert rel_err(w, oracle_w(u, 0)) < 1e-12
+
+
+def test_imported_configurations_noexp(tmp_path, wl):
+    """-noexp: per-process files loc_dir/<name>n<no>_0 in the imported format
+    (main/ms10.c:706-713), one process owning the lattice; the log says so."""
+    L = wl.lib()
+    d = str(tmp_path)
+    os.makedirs(f"{d}/loc")
+    L.set_lattice_sizes(*DIMS)
+    L.set_bc_type(3)
+    L.geometry()
+    for nc in (2, 4):
+        L.random_cnfg(ctypes.c_ulonglong(200 + nc), ctypes.c_double(0.35))
+        L.write_cnfg(f"{d}/loc/impn{nc}_0".encode())
+    write_input(f"{d}/in", d, name="imp", first=2, last=4, step=2, bc=3, lattice=None, nss=NSS,
+                mwlt=T, mwlr=R, synthetic=None)
+    txt = open(f"{d}/in").read().replace("[Directories]\n", f"[Directories]\nloc_dir      {d}/loc\n")
+    open(f"{d}/in", "w").write(txt)
+    r = run(f"{d}/in", d, "-noexp")
+    log = (tmp_path / "log" / "imp.ms10.log").read_text()
+    assert r.returncode == 0, log
+    assert "Configurations are read in imported file format" in log
+    _, recs = read_dat(f"{d}/dat/imp.ms10.dat", WLS)
+    assert [nc for nc, _ in recs] == [2, 4]
+    for nc, w in recs:
+        assert rel_err(w, oracle_w(O.generate(DIMS, 3, 200 + nc, 0.35), 3)) < 1e-12

@@ -7,6 +7,14 @@
  * archive.c (not part of this repository): header of 4 int32 lattice sizes
  * and the average plaquette, then the 8 links stored at each odd point, odd
  * points in lexicographic order, little endian.
+ *
+ * The imported (per-process) format of write_cnfg / read_cnfg, used by ms10
+ * -noexp (main/ms10.c:706-713), is the memory image of one process's field:
+ * 16 native ints (the process grid NPROC0..3, the local sizes L0..3, the
+ * process block sizes and the process coordinates cpr[0..3]), the local
+ * plaquette sum (double) and the 4*VOLUME links of udfld() in its own layout,
+ * native byte order.  Also written from openQCD-1.4's documentation; with one
+ * process owning the lattice the grid is 1x1x1x1.
  */
 #include <math.h>
 #include <stdint.h>
@@ -116,3 +124,67 @@ void random_cnfg(unsigned long long seed, double eps)
    wl_check(wlg_generate(c, (uint64_t)seed, eps), "random_cnfg [archive.c]");
    wl_device_changed();
 }
+
+/* header of an imported (per-process) configuration file */
+static void imported_head(int ldat[16])
+{
+   int h[16] = {NPROC0, NPROC1, NPROC2, NPROC3, L0, L1, L2, L3,
+                NPROC0_BLK, NPROC1_BLK, NPROC2_BLK, NPROC3_BLK, cpr[0], cpr[1], cpr[2], cpr[3]};
+   memcpy(ldat, h, sizeof(h));
+}
+
+void write_cnfg(const char *out)
+{
+   su3_dble *ud = udfld();
+   double plaq = lnk_plaq_sum(ud);
+   ud = udfld();
+   int ldat[16];
+   imported_head(ldat);
+   FILE *f = fopen(out, "wb");
+   error_root(f == NULL, 1, "write_cnfg [archive.c]", "Unable to open output file %s", out);
+   size_t iw = fwrite(ldat, sizeof(int), 16, f);
+   iw += fwrite(&plaq, sizeof(double), 1, f);
+   iw += fwrite(ud, sizeof(su3_dble), 4 * (size_t)VOLUME, f);
+   fclose(f);
+   error_root(iw != 17 + 4 * (size_t)VOLUME, 1, "write_cnfg [archive.c]", "Incorrect write count");
+}
+
+void cnfg_sizes_imported(const char *in, int n[4])
+{
+   FILE *f = fopen(in, "rb");
+   error_root(f == NULL, 1, "cnfg_sizes_imported [archive.c]", "Unable to open configuration file %s",
+              in);
+   int ldat[16];
+   size_t ir = fread(ldat, sizeof(int), 16, f);
+   fclose(f);
+   error_root(ir != 16, 1, "cnfg_sizes_imported [archive.c]", "Incorrect read count (%s)", in);
+   error_root(ldat[0] != 1 || ldat[1] != 1 || ldat[2] != 1 || ldat[3] != 1, 1,
+              "cnfg_sizes_imported [archive.c]",
+              "%s was written on a %dx%dx%dx%d process grid; one process owns the lattice here", in,
+              ldat[0], ldat[1], ldat[2], ldat[3]);
+   for (int m = 0; m < 4; m++) n[m] = ldat[4 + m];
+}
+
+void read_cnfg(const char *in)
+{
+   FILE *f = fopen(in, "rb");
+   error_root(f == NULL, 1, "read_cnfg [archive.c]", "Unable to open configuration file %s", in);
+   int ldat[16], want[16];
+   imported_head(want);
+   size_t ir = fread(ldat, sizeof(int), 16, f);
+   error_root(ir != 16 || memcmp(ldat, want, sizeof(want)) != 0, 1, "read_cnfg [archive.c]",
+              "Lattice or process grid of %s does not match", in);
+   double plaq0;
+   ir += fread(&plaq0, sizeof(double), 1, f);
+   su3_dble *ud = udfld();
+   ir += fread(ud, sizeof(su3_dble), 4 * (size_t)VOLUME, f);
+   fclose(f);
+   error_root(ir != 17 + 4 * (size_t)VOLUME, 1, "read_cnfg [archive.c]", "Incorrect read count (%s)",
+              in);
+   set_flags(UPDATED_UD);
+   double plaq1 = lnk_plaq_sum(udfld());
+   double eps = sqrt(6.0 * (double)VOLUME) * DBL_EPSILON * fabs(plaq0);
+   error_root(fabs(plaq1 - plaq0) > eps, 1, "read_cnfg [archive.c]", "Incorrect plaquette sum in %s",
+              in);
+}
+
