@@ -73,6 +73,15 @@ int wlg_set_bc(wlg_ctx *ctx);
 /* sum over x and planes of Re tr U_mu(x)U_nu(x+mu)(U_nu(x)U_mu(x+nu))^dag, time
  * plaquettes at x0 = N0-1 left out for open bc (lnkplaq.c:68-128) */
 int wlg_plaquette(wlg_ctx *ctx, double *sum);
+/* wlg_plaquette that allocates no memory.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * scratch (the per-workgroup partial sums, at most 1024, and the total) is a
+ * fixed static device array of 2048 doubles per GPU, held until the process
+ * ends; a lattice whose plaquette would need more is rejected with an error.
+ * Calls that overlap on one GPU share that array and race: serialise them.
+ * Same result, bit for bit, as wlg_plaquette. */
+int wlg_plaquette_noalloc(wlg_ctx *ctx, double *sum);
 /* v_mu += S_{mu nu}[a,b] on host AoS fields [V][4][18] (staple_sum.c:53-87) */
 int wlg_staple_sum(wlg_ctx *ctx, const double *a, const double *b, int mu, int nu, double *v);
 
