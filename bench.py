@@ -14,6 +14,15 @@ contraction and copies W (3.1 MB) to the host.
 Paths: "mixed" (FP32 staples and transporter products, FP64 projection, FP64
 time-transporter chain beyond T = 8, FP64 sums) is the headline value; the
 plain FP64 path is timed beside it, and the mixed W is compared with it.
+
+Reported beside the value: one roofline entry per launch kind of each path
+("roofline_by_kernel"; "roofline" is the kind that takes the most time of the
+mixed path), the CPU oracle on this host per build and thread count
+("cpu_baseline"), the upload through the drop-in boundary ("host_input"), a
+short serving stream in both precisions ("serving") and the module steps
+outside the headline path ("aux_kernels").  --workload c4 measures the
+16-lattice batch instead; --cpu-baseline-full times whole 8^4 and 24^3x48
+oracle measurements.
 """
 import argparse
 import json
@@ -68,30 +77,167 @@ def time_path(lat, parms, precision, steps, warmup):
         lat_ms.append((time.perf_counter() - ts) * 1e3)
     lat.synchronize()
     total = time.perf_counter() - t0
-    return w, total, lat_ms, lat.timers(), lat.contract_stats()
+    return w, total, lat_ms, lat.timers(), lat.launch_stats()
+
+
+def hyp_flop_per_site(p):
+    """48 staple sums (4 products and the accumulation each) and 28
+    combinations (1-a)U + bS with the projection (55 + 525 p flop)"""
+    return 48 * (4 * 198 + 35) + 28 * (56 + 55 + 525 * p)
+
+
+def ape_flop_per_site(p):
+    """3 spatial links: 8 staple products, 108 flop of accumulation and
+    combination, the projection"""
+    return 3 * (8 * 198 + 108 + 55 + 525 * p)
+
+
+KERNEL_LABELS = {
+    ("ape_step", "mixed"): "ape_step<float, float>",
+    ("ape_step", "fp64"): "ape_step<double, double>",
+    ("hyp", "mixed"): "hyp_level1/2/3<float, double> (3 launches)",
+    ("hyp", "fp64"): "hyp_level1/2/3<double, double> (3 launches)",
+    ("fused", "mixed"): "contract_depth<4, 8, float, float> (fused time slices, FP32 blocks)",
+    ("fused", "fp64"): "contract_depth<4, 8, double, double> (fused time slices)",
+    ("chain", "mixed"): "contract_chain1<4, float, float, false> (one-lane chain)",
+    ("chain_build", "mixed"): "contract_chain1<4, float, float, true> (one-lane chain forming S_R)",
+    ("chain", "fp64"): "contract_chain2<4, double, double, *, false> (two-lane chain)",
+    ("chain_build", "fp64"): "contract_chain2<4, double, double, *, true> (two-lane chain forming S_R)",
+}
+
+
+def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile):
+    """One roofline entry per launch kind of a path: the APE step and HYP
+    against the vector FLOP peak (FP32 without packed math and FP64 issue at
+    the same rate), the fused time-slice launches against the vector FLOP peak,
+    the per-(R,T) chain launches and the chain launch that forms S_R against
+    HBM.  achieved = algorithmic work per launch / average launch time (HIP
+    events on the context's stream around each group of launches; the APE step
+    from the levels stage over its step count, HYP from its stage per
+    configuration); traffic = the HBM bytes per launch of the committed rocprof
+    summary, where it has the kind."""
+    link = 72 if prec == "mixed" else 144
+    msl = parms.msl
+    nsteps = sum(parms.nss[:msl])
+    pairs = parms.mwlr * parms.mwlt
+    traffic = profile.get("bytes_per_launch", {})
+    out = []
+
+    def entry(kind, per_launch, unit_kind, launches_per_cfg, ms_per_launch, key=None):
+        if ms_per_launch <= 0:
+            return
+        if unit_kind == "flop":
+            ach = per_launch / (ms_per_launch * 1e-3) / 1e12
+            e = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": ach / VALU_PEAK_TFLOPS, "algorithmic_flop_per_launch": per_launch}
+        else:
+            ach = per_launch / (ms_per_launch * 1e-3) / 1e9
+            e = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": per_launch}
+        tb = traffic.get(key or f"{kind}/{prec}")
+        e.update({"traffic": tb, "kernel": KERNEL_LABELS.get((kind, prec), kind), "kind": kind,
+                  "avg_launch_ms": ms_per_launch, "launches_per_config": launches_per_cfg,
+                  "ms_per_config": ms_per_launch * launches_per_cfg})
+        if tb:
+            e["measured_hbm_GBps"] = tb / (ms_per_launch * 1e-3) / 1e9
+        out.append(e)
+
+    if nsteps:
+        entry("ape_step", V * ape_flop_per_site(parms.proj_iter), "flop", nsteps,
+              timers["levels"] / steps / nsteps)
+    entry("hyp", V * hyp_flop_per_site(parms.proj_iter), "flop", 1, timers["hyp"] / steps)
+    n_chain = lstats["chain"][0] / steps
+    n_build = lstats["chain_build"][0] / steps
+    n_fused = lstats["fused"][0] / steps
+    if n_fused:
+        t_per_launch = (pairs - n_chain - n_build) / n_fused
+        flop = V * 3 * (2 * msl * 198 + msl * msl * 36) * t_per_launch
+        entry("fused", flop, "flop", n_fused, lstats["fused"][1] / lstats["fused"][0])
+    if n_chain:
+        entry("chain", V * (1 + 3 * msl) * link, "bytes", n_chain, lstats["chain"][1] / lstats["chain"][0])
+    if n_build:
+        # L_T, F and S_{R-1} read, S_R written (the sinks are formed, not read)
+        entry("chain_build", V * (1 + 9 * msl) * link, "bytes", n_build,
+              lstats["chain_build"][1] / lstats["chain_build"][0])
+    return out
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))  # the host's CPU share for this job
+    return model, n
 
 
 def cpu_baseline(wl_cfg, parms_o):
-    """Oracle (the reference loop structure, FP64) timed on this host's cores on a
-    48^3 x 32 slab of the same workload (one HYP, one APE step, one L_T and one
-    S_R update, one (R,T) contraction: about 10 s of 16-core work; the oracle
-    spreads its threads over time slices, so the slab has at least as many
-    slices as threads), extrapolated per site and per stage count."""
+    """The oracle (the reference loop structure, FP64) timed on this host, per
+    build and thread count, on bounded slabs of the workload (one HYP, one APE
+    step, one L_T and one S_R update, one (R,T) contraction each), extrapolated
+    per site and per stage count to the whole configuration:
+      reference flags (-O -mno-avx, as main/Makefile:86-88), 1 thread: 24^3x4 slab;
+      reference flags, all host threads: 48^3x32 slab;
+      -O3 -march=x86-64-v3, 1 thread: 24^3x4 slab;
+      -O3 -march=x86-64-v3, all host threads: 48^3x32 slab.
+    The oracle spreads its threads over time slices, so the all-thread slabs
+    have at least as many slices as threads.  About 30-60 s of CPU work."""
     import oracle as O
     dims = wl_cfg["dims"]
-    sdims = (min(32, dims[0]),) + tuple(dims[1:])
-    u = O.generate(sdims, 3, wl_cfg["seed"], wl_cfg["eps"])
-    cores = min(16, len(os.sched_getaffinity(0)))
-    t = O.time_stages(sdims, 3, u, ALPHA_ACTION, parms_o, cores)
-    f = float(np.prod(dims)) / float(np.prod(sdims))
+    model, ncpu = host_cpu()
     steps = sum(wl_cfg["nss"])
     R, T = wl_cfg["R"], wl_cfg["T"]
-    t_cfg = f * (t[0] + steps * t[1] + R * (T - 1) * t[2] + (R - 1) * t[3] + R * T * t[4])
-    sample = (f"oracle stages on a {sdims[1]}^3x{sdims[0]} slab ({t.sum():.1f} s: 1 HYP, 1 APE step, "
-              f"1 L_T update, 1 S_R update, 1 (R,T) contraction), scaled by {f:.0f}x volume and "
-              f"stage counts ({steps} APE steps, {R}x{T - 1} L_T, {R - 1} S_R, {R * T} contractions)")
-    return {"value": 1.0 / t_cfg, "unit": "configs/s", "cores": cores, "kind": "port",
-            "sample": sample, "seconds_per_config": t_cfg}
+    out = {}
+    for variant, label in (("refflags", "reference_flags_O_mno_avx"), ("v3", "O3_x86_64_v3")):
+        res = {}
+        for threads, sdims in ((1, (4,) + (min(24, dims[1]),) * 3), (ncpu, (min(max(32, ncpu), dims[0]),) + tuple(dims[1:]))):
+            u = O.generate(sdims, 3, wl_cfg["seed"], wl_cfg["eps"])
+            t = O.time_stages(sdims, 3, u, ALPHA_ACTION, parms_o, threads, variant=variant)
+            f = float(np.prod(dims)) / float(np.prod(sdims))
+            t_cfg = f * (t[0] + steps * t[1] + R * (T - 1) * t[2] + (R - 1) * t[3] + R * T * t[4])
+            res[f"threads_{threads}"] = {"configs_per_s": 1.0 / t_cfg, "seconds_per_config": t_cfg,
+                                         "slab": "x".join(map(str, sdims[::-1])), "sample_s": float(t.sum())}
+            log(f"[bench] cpu baseline {label} {threads} threads: {t_cfg:.1f} s per config ({t.sum():.1f} s sample)")
+        out[label] = res
+    best = out["O3_x86_64_v3"][f"threads_{ncpu}"]
+    return {"value": best["configs_per_s"], "unit": "configs/s", "cores": ncpu, "kind": "port",
+            "sample": (f"oracle stages (1 HYP, 1 APE step, 1 L_T update, 1 S_R update, 1 (R,T) "
+                       f"contraction) on a {best['slab']} slab, -O3 -march=x86-64-v3 build, {ncpu} "
+                       f"threads, extrapolated per site and stage count ({steps} APE steps, "
+                       f"{R}x{T - 1} L_T, {R - 1} S_R, {R * T} contractions); the other builds and "
+                       f"thread counts in 'builds'"),
+            "seconds_per_config": best["seconds_per_config"], "host_cpu": model, "builds": out}
+
+
+def cpu_baseline_full(log_out):
+    """Whole ms10 measurements of the 8^4 and 24^3x48 workloads by the oracle
+    (reference loop structure), per build and thread count (the 24^3x48 one at
+    one thread is left out: about 30 minutes)."""
+    import oracle as O
+    model, ncpu = host_cpu()
+    res = {"host_cpu": model, "threads_all": ncpu, "runs": []}
+    for name in ("c1", "c2"):
+        cfg = WORKLOADS[name]
+        p = O.make_parms(cfg["nss"], cfg["T"], cfg["R"], alpha_ape=ALPHA_APE, proj_iter=PROJ_ITER)
+        u = O.generate(cfg["dims"], cfg["bc"], cfg["seed"], cfg["eps"])
+        for variant in ("refflags", "v3"):
+            for threads in (1, ncpu):
+                if name == "c2" and threads == 1:
+                    continue
+                t, _ = O.ms10_timed(cfg["dims"], cfg["bc"], u, ALPHA_ACTION, p, threads, variant=variant)
+                r = {"workload": cfg["name"], "build": variant, "threads": threads, "seconds": t,
+                     "configs_per_s": 1.0 / t}
+                log_out(f"[bench] cpu full {name} {variant} {threads} threads: {t:.2f} s")
+                res["runs"].append(r)
+    return res
 
 
 # flop per site of the smearing steps outside the headline path (M = 198 flop per
@@ -130,6 +276,22 @@ def aux_kernels(lat, steps):
             d.update({"flop_per_site": flop, "achieved_tflops": tf, "frac_of_fp64_vector_peak": tf / VALU_PEAK_TFLOPS})
         out[name] = d
         log(f"[bench] {name}: {d}")
+    return out
+
+
+def serving_leg(n, sizes):
+    """The serving form: a Poisson stream (mean 4 requests/s) of n seeded
+    configurations per precision, sizes uniform over 12^3x24 .. 64^3x128, half
+    open and half periodic, each HYP + APE levels {0,8,24,48} + W up to L/2,
+    admitted by HBM footprint under a 260 GB budget (the lean layout where the
+    default one does not fit); configs/s and p50/p99 latency per precision."""
+    import wloop_amd as W
+    from wloop_amd import serve
+    out = {}
+    for prec, name in ((W.MIXED, "mixed"), (W.FP64, "fp64")):
+        res, _ = serve.run_stream(4.0, n, prec, 260.0, workers=2, sizes=sizes, seed=21, log=log)
+        out[name] = res
+        log(f"[bench] serving {name}: {res}")
     return out
 
 
@@ -203,7 +365,17 @@ def main():
                     help="keep the library's default launch shapes (for profiling runs)")
     ap.add_argument("--batch", type=int, default=2,
                     help="configurations per captured graph in the batched leg (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12,
+                    help="requests per precision in the serving leg (0: skip)")
+    ap.add_argument("--serve-sizes", default="12,16,24,32,48,64",
+                    help="spatial extents L of the serving leg's L^3 x 2L lattices")
+    ap.add_argument("--cpu-baseline-full", action="store_true",
+                    help="only time whole 8^4 and 24^3x48 measurements of the CPU oracle per build "
+                         "and thread count, print them as one JSON line and exit")
     args = ap.parse_args()
+    if args.cpu_baseline_full:
+        print(json.dumps(cpu_baseline_full(log)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not "
               "part of this build: independent configurations shard one per GPU with no "
@@ -247,14 +419,14 @@ def main():
         autotune.apply(lat, plain, W.FP64)
     log(f"[bench] autotuned contraction: {tuned}")
 
-    w64, t64, lat64, tim64, cs64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
+    w64, t64, lat64, tim64, ls64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
     log(f"[bench] fp64 (plain): {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
     best64 = None
     if args.no_autotune or choice["fp64"]["kernel"] == W.KERNEL_DEPTH:
         # the fused time-slice form for FP64 (library default shape without autotuning)
         autotune.apply(lat, {"kernel": W.KERNEL_DEPTH, "wg": 0} if args.no_autotune else choice["fp64"],
                        W.FP64)
-        w64b, t64b, lat64b, tim64b, cs64b = time_path(lat, parms, W.FP64, args.steps, args.warmup)
+        w64b, t64b, lat64b, tim64b, ls64b = time_path(lat, parms, W.FP64, args.steps, args.warmup)
         log(f"[bench] fp64 (fused time slices): {t64b / args.steps * 1e3:.1f} ms/config  stages {tim64b}")
         best64 = {"value": args.steps / t64b, "ms_per_step": t64b / args.steps * 1e3,
                   "contraction": tuned["fp64"] if choice else "fused time slices, library default shape",
@@ -262,7 +434,7 @@ def main():
                                  "p99": float(np.percentile(lat64b, 99))},
                   "stage_ms_per_config": {k: v / args.steps for k, v in tim64b.items()},
                   "max_rel_err_vs_plain": rel_err(w64b, w64)}
-    wmx, tmx, latmx, timmx, csmx = time_path(lat, parms, W.MIXED, args.steps, args.warmup)
+    wmx, tmx, latmx, timmx, lsmx = time_path(lat, parms, W.MIXED, args.steps, args.warmup)
     log(f"[bench] mixed: {tmx / args.steps * 1e3:.1f} ms/config  stages {timmx}")
     err = rel_err(wmx, w64)
     log(f"[bench] mixed vs fp64 max rel err {err:.3e}")
@@ -316,64 +488,17 @@ def main():
     log(f"[bench] host input: {host_input}")
 
     V = int(np.prod(dims))
-    links_per_site = 1 + 3 * parms.msl            # L_T + S_R of every level and direction
-
-    prof = os.path.join(ROOT, "profiles", "contract_traffic.json")
-    traffic = {}
-    if os.path.exists(prof):
-        with open(prof) as f:
-            traffic = json.load(f).get("bytes_per_launch", {})
-
-    def roof(cs, prec_name, kernel):
-        """roofline of the contraction launches of one path: HBM bytes for the
-        per-(R,T) forms, FLOPs for the fused time-slice form"""
-        n, ms = cs
-        avg_s = ms / n / 1e3
-        if kernel == W.KERNEL_DEPTH:
-            # per (x,k,T): 2 msl 3x3 products (198 flop) and msl^2 traces (36 flop)
-            flop = (args.steps * V * 3 * parms.mwlt * parms.mwlr *
-                    (2 * parms.msl * 198 + parms.msl ** 2 * 36) / n)
-            ach = flop / avg_s / 1e12
-            return {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / VALU_PEAK_TFLOPS, "traffic": traffic.get(f"{kernel}/{prec_name}"),
-                    "kernel": W.KERNEL_NAMES[kernel], "avg_launch_ms": avg_s * 1e3,
-                    "algorithmic_flop_per_launch": flop}
-        link_bytes = 72 if prec_name == "mixed" else 144
-        algo = V * links_per_site * link_bytes
-        if kernel == W.KERNEL_HYBRID and prec_name == "mixed":
-            # two launch kinds (one fused launch per R and block of T <= 8, one
-            # one-lane chain launch per (R,T) beyond; the first chain launch of
-            # each R >= 2 also forms S_R = F S_{R-1}(x+k): 3 x 3 msl links per
-            # site): the unit is one (R,T) pair
-            n_rt = parms.mwlt * parms.mwlr
-            per_rt = ms / (args.steps * n_rt) / 1e3
-            algo_rt = algo + (parms.mwlr - 1) * V * 9 * parms.msl * link_bytes / n_rt
-            nf = min(parms.mwlt, 8)
-            tf, tc = traffic.get(f"{W.KERNEL_DEPTH}/mixed"), traffic.get(f"{W.KERNEL_CHAIN1}/mixed")
-            tb = traffic.get(f"{W.KERNEL_CHAIN1}b/mixed")  # chain launch forming S_R
-            tr = None
-            if tf and tc and tb and parms.mwlt > nf:
-                per_r = [tf + tc * (parms.mwlt - nf) if r == 0 else tf + tb + tc * (parms.mwlt - nf - 1)
-                         for r in range(parms.mwlr)]
-                tr = sum(per_r) / n_rt
-            ach = algo_rt / per_rt / 1e9
-            return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": ach / HBM_PEAK_GBS, "traffic": tr, "kernel": W.KERNEL_NAMES[kernel],
-                    "basis": "per (R,T) pair over both launch kinds, spatial transporters formed "
-                             "in the first chain launch of each R included",
-                    "avg_ms_per_rt": per_rt * 1e3, "algorithmic_bytes_per_rt": algo_rt}
-        ach = algo / avg_s / 1e9
-        return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": traffic.get(f"{kernel}/{prec_name}"),
-                "kernel": W.KERNEL_NAMES[kernel], "avg_launch_ms": avg_s * 1e3,
-                "algorithmic_bytes_per_launch": algo}
-
-    k_mx = choice["mixed"]["kernel"] if choice else W.KERNEL_HYBRID
-    k_64 = plain["kernel"] if choice else W.KERNEL_CHAIN2
-    roof_mx = roof(csmx, "mixed", k_mx)
-    roof_64 = roof(cs64, "fp64", k_64)
+    prof_path = os.path.join(ROOT, "profiles", "kernel_profile_48c96_open.json")
+    profile = {}
+    if os.path.exists(prof_path) and args.workload == "c3":
+        with open(prof_path) as f:
+            profile = json.load(f)
+    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile),
+                 "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, profile)}
     if best64:
-        best64["roofline"] = roof(cs64b, "fp64", W.KERNEL_DEPTH)
+        rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps, profile)
+        best64["roofline"] = max(rooflines["fp64_best"], key=lambda e: e["ms_per_config"])
+    primary = max(rooflines["mixed"], key=lambda e: e["ms_per_config"])
 
     out = {
         "metric": "configs/s at 48^3x96 (mixed vs FP64 GPU, rel err<1e-6); p50/p99 latency",
@@ -396,7 +521,7 @@ def main():
         "fp64": {"value": args.steps / t64, "ms_per_step": t64 / args.steps * 1e3,
                  "latency_ms": {"p50": float(np.percentile(lat64, 50)), "p99": float(np.percentile(lat64, 99))},
                  "stage_ms_per_config": {k: v / args.steps for k, v in tim64.items()},
-                 "roofline": roof_64},
+                 "roofline": max(rooflines["fp64"], key=lambda e: e["ms_per_config"])},
         "mixed_over_fp64": t64 / tmx,
         "fp64_best": best64,
         "mixed_over_fp64_best": (best64["ms_per_step"] / (tmx / args.steps * 1e3)) if best64 else t64 / tmx,
@@ -405,7 +530,8 @@ def main():
         "mixed_max_rel_err_vs_fp64": err,
         "host_input": host_input,
         "stage_ms_per_config": {k: v / args.steps for k, v in timmx.items()},
-        "roofline": roof_mx,
+        "roofline": primary,
+        "roofline_by_kernel": rooflines,
     }
     if not args.no_cpu_baseline:
         po = None
@@ -417,6 +543,8 @@ def main():
         out["cpu_baseline"] = None
     out["aux_kernels"] = aux_kernels(lat, 5)
     lat.close()
+    sizes = tuple(int(x) for x in args.serve_sizes.split(","))
+    out["serving"] = serving_leg(args.serve_requests, sizes) if args.serve_requests > 0 else None
     print(json.dumps(out), flush=True)
 
 

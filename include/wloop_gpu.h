@@ -117,8 +117,10 @@ int wlg_batch_run(wlg_batch *batch, double *const *wl);
 double wlg_batch_ms(wlg_batch *batch);   /* device time of the last run */
 void wlg_batch_destroy(wlg_batch *batch);
 
-/* device bytes one measurement needs (admission by HBM footprint) */
+/* device bytes one measurement needs (admission by HBM footprint), with the
+ * default layout and with the lean layout (WLG_TUNE_LEAN = 1) */
 size_t wlg_footprint(const int dims[4], const wlg_parms *p, int precision);
+size_t wlg_footprint_lean(const int dims[4], const wlg_parms *p, int precision);
 
 /* launch-shape parameters of a context (0 = library default).  For each
  * precision: the contraction kernel (1 streaming form, 2 chain form with msl
@@ -139,7 +141,12 @@ enum {
    WLG_TUNE_MIXED_CONTRACT_KERNEL, WLG_TUNE_MIXED_CONTRACT_WG,
    WLG_TUNE_FP64_CONTRACT_DEPTH, WLG_TUNE_FP64_CONTRACT_STARTS,
    WLG_TUNE_MIXED_CONTRACT_DEPTH, WLG_TUNE_MIXED_CONTRACT_STARTS,
-   WLG_TUNE_APE_FORM,   /* 1: one kernel per APE step, 2: staple sums, then lincomb and projection */
+   WLG_TUNE_APE_FORM,   /* APE kernels: 1 one thread per link, 2 per-link staple sums then
+                           lincomb and projection, 3 one thread per site, 4 per-site staple
+                           sums then lincomb and projection */
+   WLG_TUNE_LEAN,       /* 1: least device memory (wlg_footprint_lean): fused time-slice
+                           contraction for every T, time transporters kept only at the
+                           starts of its T blocks, APE steps through the level buffers */
    WLG_TUNE_COUNT
 };
 int wlg_tune(wlg_ctx *ctx, int param, int value);
@@ -153,6 +160,12 @@ double wlg_timer_ms(wlg_ctx *ctx, int stage);
 /* number of contraction launches and their summed duration (ms), measured with
  * HIP events on the context's stream */
 int wlg_contract_stats(wlg_ctx *ctx, long *launches, double *ms);
+/* the same by launch kind: the fused time-slice launches, the per-(R,T) chain
+ * (or streaming) launches, the chain launch of each R >= 2 that also forms its
+ * spatial transporters, and the finish launch of each R (its launch count
+ * and summed duration, measured with HIP events around each group of launches) */
+enum { WLG_L_FUSED = 0, WLG_L_CHAIN, WLG_L_CHAIN_BUILD, WLG_L_FINISH, WLG_L_COUNT };
+int wlg_launch_stats(wlg_ctx *ctx, int kind, long *launches, double *ms);
 
 #ifdef __cplusplus
 }

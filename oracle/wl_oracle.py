@@ -44,35 +44,52 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def lib():
+_variants = {}
+
+
+def lib(variant=None):
+    """the parity oracle, or a CPU-baseline build of the same source
+    (variant "refflags" or "v3", see oracle/Makefile)"""
     global _lib
+    if variant is not None:
+        if variant not in _variants:
+            path = os.path.join(_HERE, f"liboracle_{variant}.so")
+            if not os.path.exists(path):
+                build()
+            _variants[variant] = _bind(ctypes.CDLL(path))
+        return _variants[variant]
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        _lib = ctypes.CDLL(_LIB)
-        dp = ctypes.POINTER(ctypes.c_double)
-        ip = ctypes.POINTER(ctypes.c_int)
-        _lib.wo_generate.argtypes = [ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, dp]
-        _lib.wo_project.argtypes = [dp, ctypes.c_int]
-        _lib.wo_staple_sum.argtypes = [ip, dp, dp, ctypes.c_int, ctypes.c_int, dp]
-        _lib.wo_hyp.argtypes = [ip, ctypes.c_int, dp] + [ctypes.c_double] * 3 + [ctypes.c_int] * 4
-        _lib.wo_hyp_spatial.argtypes = [ip, dp, ctypes.c_double, ctypes.c_double,
-                                        ctypes.c_int, ctypes.c_int]
-        _lib.wo_ape.argtypes = [ip, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        _lib.wo_wloop.argtypes = [ip, dp, ctypes.POINTER(Parms), ctypes.c_int, ctypes.c_int, dp]
-        _lib.wo_ms10.argtypes = [ip, ctypes.c_int, dp, dp, ctypes.POINTER(Parms),
-                                 ctypes.c_int, ctypes.c_int, dp]
-        _lib.wo_levels.argtypes = [ip, dp, ctypes.POINTER(Parms), dp]
-        _lib.wo_gauge_transform.argtypes = [ip, dp, ctypes.c_uint64]
-        _lib.wo_shift.argtypes = [ip, dp, ip]
-        _lib.wo_plaquette.argtypes = [ip, dp]
-        _lib.wo_plaquette.restype = ctypes.c_double
-        _lib.wo_random_su3.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, dp]
-        _lib.wo_philox4x32.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
-        _lib.wo_set_threads.argtypes = [ctypes.c_int]
-        _lib.wo_time_stages.argtypes = [ip, ctypes.c_int, dp, dp, ctypes.POINTER(Parms),
-                                        ctypes.c_int, dp]
+        _lib = _bind(ctypes.CDLL(_LIB))
     return _lib
+
+
+def _bind(L):
+    """argument and result types of the oracle's C functions"""
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.wo_generate.argtypes = [ip, ctypes.c_int, ctypes.c_uint64, ctypes.c_double, dp]
+    L.wo_project.argtypes = [dp, ctypes.c_int]
+    L.wo_staple_sum.argtypes = [ip, dp, dp, ctypes.c_int, ctypes.c_int, dp]
+    L.wo_hyp.argtypes = [ip, ctypes.c_int, dp] + [ctypes.c_double] * 3 + [ctypes.c_int] * 4
+    L.wo_hyp_spatial.argtypes = [ip, dp, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_int, ctypes.c_int]
+    L.wo_ape.argtypes = [ip, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.wo_wloop.argtypes = [ip, dp, ctypes.POINTER(Parms), ctypes.c_int, ctypes.c_int, dp]
+    L.wo_ms10.argtypes = [ip, ctypes.c_int, dp, dp, ctypes.POINTER(Parms),
+                             ctypes.c_int, ctypes.c_int, dp]
+    L.wo_levels.argtypes = [ip, dp, ctypes.POINTER(Parms), dp]
+    L.wo_gauge_transform.argtypes = [ip, dp, ctypes.c_uint64]
+    L.wo_shift.argtypes = [ip, dp, ip]
+    L.wo_plaquette.argtypes = [ip, dp]
+    L.wo_plaquette.restype = ctypes.c_double
+    L.wo_random_su3.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, dp]
+    L.wo_philox4x32.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+    L.wo_set_threads.argtypes = [ctypes.c_int]
+    L.wo_time_stages.argtypes = [ip, ctypes.c_int, dp, dp, ctypes.POINTER(Parms),
+                                    ctypes.c_int, dp]
+    return L
 
 
 def _dp(a):
@@ -208,11 +225,24 @@ def from_complex(c):
     return out
 
 
-def time_stages(dims, bc, u, alpha_action, parms, nthreads=0):
+def time_stages(dims, bc, u, alpha_action, parms, nthreads=0, variant=None):
     """Seconds of one HYP, one APE step, one L_T update, one S_R update (all
     levels) and one (R,T) contraction of the reference loop structure."""
     t = np.zeros(5)
     aa = np.asarray(alpha_action, dtype=np.float64)
-    lib().wo_time_stages(_dims(dims), bc, _dp(np.ascontiguousarray(u)), _dp(aa),
-                         ctypes.byref(parms), nthreads, _dp(t))
+    lib(variant).wo_time_stages(_dims(dims), bc, _dp(np.ascontiguousarray(u)), _dp(aa),
+                                ctypes.byref(parms), nthreads, _dp(t))
     return t
+
+
+def ms10_timed(dims, bc, u, alpha_action, parms, nthreads, variant=None, mode=0):
+    """Wall seconds of one full ms10 measurement (HYP + wloop_sum, the reference
+    loop structure with mode 0) with a given build and thread count."""
+    import time
+    u = u.copy()
+    n = parms.msl ** 2 * parms.mwlt * parms.mwlr * dims[0]
+    wl = np.empty(n)
+    aa = np.asarray(alpha_action, dtype=np.float64)
+    t0 = time.perf_counter()
+    lib(variant).wo_ms10(_dims(dims), bc, _dp(u), _dp(aa), ctypes.byref(parms), mode, nthreads, _dp(wl))
+    return time.perf_counter() - t0, wl

@@ -122,18 +122,51 @@ def test_autotune_picks_a_candidate(wl, tmp_path):
 
 @pytest.mark.parametrize("precision", [0, 1])
 def test_ape_forms_agree(wl, precision):
-    """APE as one kernel or as staple sums then projection: the same arithmetic,
-    so the same bits."""
+    """APE with one thread per link (1) or per site (3), each as one kernel or as
+    staple sums then projection (2, 4): one kernel and its two-kernel split do
+    the same arithmetic, so the same bits; every form matches the oracle.  Odd
+    and even extents, open bc."""
     dims, bc = (8, 6, 5, 7), 0
     u = O.generate(dims, bc, 46, 0.3)
     nss = [0, 2, 3]
     ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, 4, 2))
     p = wl.make_parms(nss, 4, 2)
-    outs = []
+    outs = {}
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
-        for form in (1, 2):
+        for form in (1, 2, 3, 4):
             lat.tune(wl.TUNE_APE_FORM, form)
-            outs.append(lat.measure(p, precision))
-    assert np.array_equal(outs[0], outs[1])
-    assert rel_err(outs[1], ref) < (1e-12 if precision == 0 else 1e-6)
+            outs[form] = lat.measure(p, precision)
+    assert np.array_equal(outs[1], outs[2])
+    assert np.array_equal(outs[3], outs[4])
+    for form in (1, 3):
+        assert rel_err(outs[form], ref) < (1e-12 if precision == 0 else 1e-6), form
+
+
+# lean layout (least device memory): only storage moves, so it gives the bits of
+# the fused form with the same T depth; T blocks starting past T = 1 in both
+# precisions (depth 4, T up to 11), level 0 aliasing the field (nss[0] = 0) or
+# smeared, odd and even step counts per level, and a batch of lean contexts
+@pytest.mark.parametrize("nss", [[0, 3, 2], [1, 2, 3]])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_lean_layout(wl, nss, precision):
+    dims, bc, mwlt, mwlr = (22, 5, 6, 7), 0, 11, 3
+    u = O.generate(dims, bc, 48, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    pk, pd = ((wl.TUNE_MIXED_CONTRACT_KERNEL, wl.TUNE_MIXED_CONTRACT_DEPTH) if precision else
+              (wl.TUNE_FP64_CONTRACT_KERNEL, wl.TUNE_FP64_CONTRACT_DEPTH))
+    with wl.Lattice(dims, bc) as a, wl.Lattice(dims, bc) as b:
+        a.upload(u)
+        b.upload(u)
+        a.tune(pk, wl.KERNEL_DEPTH)
+        a.tune(pd, 4)
+        b.tune(wl.TUNE_LEAN, 1)
+        b.tune(pd, 4)
+        fused = a.measure(p, precision)
+        lean = b.measure(p, precision)
+        assert np.array_equal(fused, lean)
+        assert rel_err(lean, ref) < (1e-12 if precision == 0 else 1e-6)
+        batch = wl.Batch([b], [p], precision)
+        assert np.array_equal(batch.run()[0], lean)
+        batch.close()

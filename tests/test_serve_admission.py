@@ -134,3 +134,19 @@ def test_join_waits_for_task_done():
     s.join()
     t.join()
     assert done == [0]
+
+
+def test_largest_serving_shape_is_admitted_in_both_precisions():
+    """64^3x128 (about 19 GB of FP64 links) with APE levels {0,8,24,48}, R,T <= 32:
+    the default FP64 layout needs more than 260 GB, the lean one fits."""
+    from wloop_amd.serve import context_layout
+    budget = 260e9
+    r64 = standard_request(0, 64, W.OPEN, W.FP64)
+    assert W.footprint(r64.dims, r64.parms, W.FP64) > budget
+    lean, nbytes = context_layout(r64, budget)
+    assert lean and nbytes <= budget
+    rmx = standard_request(1, 64, W.PERIODIC, W.MIXED)
+    lean, nbytes = context_layout(rmx, budget)
+    assert not lean and nbytes <= budget
+    small = standard_request(2, 16, W.OPEN, W.FP64)
+    assert context_layout(small, budget) == (False, W.footprint(small.dims, small.parms, W.FP64))

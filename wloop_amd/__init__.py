@@ -19,13 +19,15 @@ OPEN, PERIODIC = 0, 3
 T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
 (TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG,
  TUNE_FP64_CONTRACT_DEPTH, TUNE_FP64_CONTRACT_STARTS, TUNE_MIXED_CONTRACT_DEPTH,
- TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM) = range(9)
+ TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM, TUNE_LEAN) = range(10)
 KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH, KERNEL_CHAIN2, KERNEL_HYBRID = 1, 2, 3, 4, 5, 6
 KERNEL_NAMES = {KERNEL_STREAM: "streaming", KERNEL_CHAIN: "chain (msl lanes per pair)",
                 KERNEL_CHAIN1: "chain (one lane per pair)", KERNEL_DEPTH: "fused time slices",
                 KERNEL_CHAIN2: "chain (two lanes per pair)",
                 KERNEL_HYBRID: "fused time slices for T <= 8, one-lane chain beyond"}
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
+L_FUSED, L_CHAIN, L_CHAIN_BUILD, L_FINISH = range(4)
+LAUNCH_KINDS = {L_FUSED: "fused", L_CHAIN: "chain", L_CHAIN_BUILD: "chain_build", L_FINISH: "finish"}
 
 
 class WloopError(RuntimeError):
@@ -89,12 +91,16 @@ def lib():
     L.wlg_level_download.argtypes = [vp, ctypes.c_int, dp]
     L.wlg_footprint.argtypes = [ip, ctypes.POINTER(Parms), ctypes.c_int]
     L.wlg_footprint.restype = ctypes.c_size_t
+    L.wlg_footprint_lean.argtypes = [ip, ctypes.POINTER(Parms), ctypes.c_int]
+    L.wlg_footprint_lean.restype = ctypes.c_size_t
+    L.wlg_plaquette_noalloc.argtypes = [vp, dp]
     L.wlg_synchronize.argtypes = [vp]
     L.wlg_timers_reset.argtypes = [vp]
     L.wlg_timers_reset.restype = None
     L.wlg_timer_ms.argtypes = [vp, ctypes.c_int]
     L.wlg_timer_ms.restype = ctypes.c_double
     L.wlg_contract_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_long), dp]
+    L.wlg_launch_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), dp]
     L.wlg_tune.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.wlg_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp),
                                    ctypes.POINTER(Parms), ctypes.c_int]
@@ -123,8 +129,11 @@ def wls(parms, n0):
     return parms.msl ** 2 * parms.mwlt * parms.mwlr * n0
 
 
-def footprint(dims, parms, precision=FP64):
-    return lib().wlg_footprint((ctypes.c_int * 4)(*dims), ctypes.byref(parms), precision)
+def footprint(dims, parms, precision=FP64, lean=False):
+    """device bytes of one measurement: default layout, or the lean layout
+    (TUNE_LEAN = 1)"""
+    f = lib().wlg_footprint_lean if lean else lib().wlg_footprint
+    return f((ctypes.c_int * 4)(*dims), ctypes.byref(parms), precision)
 
 
 class Lattice:
@@ -240,6 +249,16 @@ class Lattice:
         ms = ctypes.c_double()
         _check(lib().wlg_contract_stats(self._h, ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+    def launch_stats(self):
+        """{kind: (launches, summed ms)} of the contraction launch kinds"""
+        out = {}
+        for kind, name in LAUNCH_KINDS.items():
+            n = ctypes.c_long()
+            ms = ctypes.c_double()
+            _check(lib().wlg_launch_stats(self._h, kind, ctypes.byref(n), ctypes.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out
 
 
 class Batch:

@@ -72,6 +72,8 @@ struct wlg_ctx {
   int tune[WLG_TUNE_COUNT];  // 0 = default
   long contract_launches;
   double contract_ms;
+  long kind_launches[WLG_L_COUNT];  // contraction launches by kind (wlg_launch_stats)
+  double kind_ms[WLG_L_COUNT];
   // graph capture: no host synchronisation, timers or allocation; W goes to
   // the pinned buffer wl_pinned
   int graph;
@@ -371,6 +373,9 @@ static const int kTimeFp64From = 9;
 
 struct Layout {
   size_t lev, tmp, src32, lt, lt32, lt64, s, total;
+  int lev_alias;  // level 0 is the input field itself (nss[0] = 0, lean layout): no storage
+  int lean;
+  bool lean_lt;   // time transporters kept only at the starts of the fused T blocks
 };
 
 // contraction variant: kernel form, its workgroup parameter and, for the
@@ -402,24 +407,40 @@ static int depth_fp64_starts(int prec, int mwlt, int depth) {
   return n;
 }
 
-static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64) {
+// Arena layout of a measurement.  nb64: FP64 time transporters kept for the
+// starts of the FP64 blocks of the fused form on the mixed path.  lean (fused
+// contraction for every T): the time transporters are kept only at the starts
+// of the fused T blocks (plus two in flight while the chain is built), the
+// APE steps of a level alternate between one 3-slot buffer and the level's own
+// buffer, and a level 0 that is the input field itself takes no storage.
+static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64, int lean,
+                           const std::vector<DepthBlock>& blocks, bool fused_all) {
   size_t s64 = slot64(L), s32 = slot32(L);
   Layout o;
   size_t off = 0;
   const size_t sl = (prec == WLG_MIXED) ? s32 : s64;  // storage of levels and transporters
-  o.lev = off; off += (size_t)3 * p->msl * sl;
-  o.tmp = off; off += (p->op_smear == 1 ? 12 : 9) * sl;  // APE: 2x3 ping-pong + 3 staple sums
+  o.lean = lean;
+  o.lev_alias = lean && p->nss[0] == 0;
+  o.lev = off; off += (size_t)3 * (p->msl - o.lev_alias) * sl;
+  // APE: 2x3 ping-pong + 3 staple sums (lean: 3, through the level buffer);
+  // 3-d HYP: its 6-slot intermediate + ping-pong
+  o.tmp = off; off += (p->op_smear == 1 ? 12 : (lean ? 3 : 9)) * sl;
+  int n32 = 0, n64 = 0;  // block starts past T = 1 that need storage of their own
+  for (const DepthBlock& b : blocks)
+    if (b.Ta > 1) (b.fp64 ? n64 : n32)++;
+  const bool lean_lt = lean && fused_all;  // every T in a fused block: starts only
   if (prec == WLG_MIXED) {
     o.src32 = off; off += 3 * s32;
-    o.lt32 = off; off += (size_t)p->mwlt * s32;
+    o.lt32 = off; off += (size_t)(lean_lt ? 3 + n32 : p->mwlt) * s32;
     o.lt64 = off; off += (size_t)(2 + nb64) * s64;
     o.lt = 0;
     o.s = off; off += (size_t)2 * 3 * p->msl * s32;
   } else {
     o.src32 = o.lt32 = o.lt64 = 0;
-    o.lt = off; off += (size_t)(p->mwlt > 1 ? p->mwlt - 1 : 0) * s64;
+    o.lt = off; off += (size_t)(lean_lt ? 2 + n64 : (p->mwlt > 1 ? p->mwlt - 1 : 0)) * s64;
     o.s = off; off += (size_t)2 * 3 * p->msl * s64;
   }
+  o.lean_lt = lean_lt;
   o.total = off;
   return o;
 }
@@ -450,6 +471,7 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   // mixed path the fused form for the FP32 blocks T <= 8 and the one-lane
   // chain form beyond
   k.kernel = tk > 0 ? tk - 1 : (mixed ? 5 : 4);
+  if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
   static const int wg_default[6] = {8, 512, 512, 128, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
@@ -468,7 +490,7 @@ static size_t partial_count(const Lat& L, const wlg_parms* p, const Contract& k)
   return (size_t)p->mwlt * L.n[0] * contract_slots(L, k.kernel, k.wg) * p->msl * p->msl;
 }
 
-extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
+static size_t footprint(const int dims[4], const wlg_parms* p, int prec, int lean) {
   Lat L;
   for (int m = 0; m < 4; m++) L.n[m] = dims[m];
   L.st[3] = 1;
@@ -476,16 +498,37 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
   L.st[1] = (long)dims[2] * dims[3];
   L.st[0] = (long)dims[1] * dims[2] * dims[3];
   L.V = (long)dims[0] * L.st[0];
-  // default launch shapes: FP64 time transporters of depth-16 blocks
-  Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16));
+  const bool fused = lean && p->msl <= 4;
+  // default launch shapes: depth-16 blocks of the fused form; partial sums per
+  // x0: the most among the library defaults and the tuning candidates
+  // (streaming form: 3 x 32 chunks; chain forms: up to 2048 persistent
+  // workgroups; fused form: up to 128 workgroups per start group)
+  const Contract k = fused ? Contract{3, 128, 16, 8} : Contract{1, 2048, 16, 8};
+  const std::vector<DepthBlock> blocks =
+      fused ? depth_blocks(prec, p->mwlt, 16) : std::vector<DepthBlock>();
+  Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16), lean, blocks, fused);
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * dims[0];
-  // partial sums per x0: the most among the library defaults and the tuning
-  // candidates (streaming form: 3 x 32 chunks; chain forms: up to 2048
-  // persistent workgroups)
-  const Contract k = {1, 2048, 16, 8};
   return 8 * slot64(L) + arena + wls * sizeof(double) + partial_count(L, p, k) * sizeof(double);
+}
+
+extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec) {
+  return footprint(dims, p, prec, 0);
+}
+
+extern "C" size_t wlg_footprint_lean(const int dims[4], const wlg_parms* p, int prec) {
+  return footprint(dims, p, prec, 1);
+}
+
+// APE kernel form of the operator levels: 1 one thread per link, 2 per-link
+// staple sums then lincomb/projection, 3 one thread per site for its three
+// links, 4 per-site staple sums then lincomb/projection
+static int ape_form(const wlg_ctx* c, int prec) {
+  (void)prec;
+  const int t = c->tune[WLG_TUNE_APE_FORM];
+  if (c->tune[WLG_TUNE_LEAN] == 1 && (t == 2 || t == 4)) return t - 1;  // no staple-sum slots
+  return t > 0 ? t : 1;
 }
 
 // operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
@@ -494,8 +537,8 @@ extern "C" size_t wlg_footprint(const int dims[4], const wlg_parms* p, int prec)
 // TS = float: mixed path (levels stored in FP32, staples in FP32, the
 // projection after every step in FP64).  src is the 3-slot spatial field.
 template <class TS>
-static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, char* levbase,
-                        char* tmpbase, const v2<TS>** lev) {
+static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, const Layout& o,
+                        char* levbase, char* tmpbase, const v2<TS>** lev) {
   const Lat& L = c->L;
   const int msl = p->msl;
   const size_t lvl = (size_t)27 * L.V;
@@ -511,20 +554,28 @@ static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, char*
       lev[sl] = cur;
       continue;
     }
+    v2<TS>* mine = levbuf + (sl - o.lev_alias) * lvl;  // this level's storage
+    const bool through_level = o.lean && p->op_smear == 0;
     for (int step = 0; step < n; step++) {
-      v2<TS>* dst = (step == n - 1) ? levbuf + sl * lvl : (step % 2 ? tmp1 : tmp0);
+      v2<TS>* dst;
+      if (through_level)  // alternate with tmp0 so that the last step lands in mine
+        dst = ((n - 1 - step) % 2 == 0) ? mine : tmp0;
+      else
+        dst = (step == n - 1) ? mine : (step % 2 ? tmp1 : tmp0);
       if (p->op_smear == 1)
         launch_hyp3<TS, TS>(L, cur, tmpD, dst, p->alpha_hyp[0], p->alpha_hyp[1], proj,
                             p->proj_iter, c->st);
       else
-        if (c->tune[WLG_TUNE_APE_FORM] == 2)
-          launch_ape_split<TS, TS>(L, cur, tmpD, dst, p->alpha_ape, proj, p->proj_iter, c->st);
-        else
-          launch_ape<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st);
+        switch (ape_form(c, sizeof(TS) == 4 ? WLG_MIXED : WLG_FP64)) {
+          case 2: launch_ape_split<TS, TS>(L, cur, tmpD, dst, p->alpha_ape, proj, p->proj_iter, 0, c->st); break;
+          case 3: launch_ape_site<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st); break;
+          case 4: launch_ape_split<TS, TS>(L, cur, tmpD, dst, p->alpha_ape, proj, p->proj_iter, 1, c->st); break;
+          default: launch_ape<TS, TS>(L, cur, dst, p->alpha_ape, proj, p->proj_iter, c->st); break;
+        }
       CKL();
       cur = dst;
     }
-    lev[sl] = levbuf + sl * lvl;
+    lev[sl] = mine;
   }
   return 0;
 }
@@ -539,10 +590,10 @@ static int run_levels(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec
     float2* src = (float2*)(A + o.src32);
     launch_d2f((long)27 * L.V, U + 9 * L.V, src, c->st);  // F_0 in FP32
     CKL();
-    if (run_levels_t<float>(c, src, p, A + o.lev, A + o.tmp, lev32)) return -1;
+    if (run_levels_t<float>(c, src, p, o, A + o.lev, A + o.tmp, lev32)) return -1;
     for (int sl = 0; sl < p->msl; sl++) c->lev32[sl] = lev32[sl];
   } else {
-    if (run_levels_t<double>(c, U + 9 * L.V, p, A + o.lev, A + o.tmp, lev64)) return -1;
+    if (run_levels_t<double>(c, U + 9 * L.V, p, o, A + o.lev, A + o.tmp, lev64)) return -1;
     for (int sl = 0; sl < p->msl; sl++) c->lev[sl] = lev64[sl];
   }
   return 0;
@@ -572,13 +623,16 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
   for (const DepthBlock& b : blocks) tfused = b.Ta + b.d - 1 > tfused ? b.Ta + b.d - 1 : tfused;
   // form 5 continues with the one-lane chain (mixed) / two-lane chain (FP64)
   const int kc = k.kernel == 5 ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
-  // with FP32 level links and the one-lane chain form for the per-(R,T)
-  // launches, the first of them forms the spatial transporters of its R (no
-  // separate pass over F, S_{R-1} and S_R)
-  const bool build = sizeof(TS) == 4 && kc == 2 && msl <= 4 && tfused < p->mwlt;
+  // with the one-lane (FP32 level links) or two-lane (FP64) chain form for the
+  // per-(R,T) launches, the first of them forms the spatial transporters of its
+  // R (no separate pass over F, S_{R-1} and S_R)
+  const bool build = ((sizeof(TS) == 4 && kc == 2) || (sizeof(TS) == 8 && kc == 4)) && msl <= 4 &&
+                     tfused < p->mwlt;
   for (int r = 0; r < p->mwlr; r++) {
     const int R = r + 1;
-    hipEvent_t e0, e1, e2;
+    // events per R: start, spatial transporters, chain launch forming S_R,
+    // fused launches, chain launches, finish (the timers read them in this order)
+    hipEvent_t e0, e1, eb, ef, ec, e2;
     if (record(c, &e0)) return -1;
     const bool build_r = build && R > 1;
     if (R == 1) {
@@ -595,14 +649,14 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     if (record(c, &e1)) return -1;
     int tnext = tfused;  // first T of the per-(R,T) launches
     if (build_r) {
-      if constexpr (sizeof(TS) == 4) {
-        launch_contract_build<TS, TC>(L, msl, R, tnext + 1, LT[tnext], lev, prev, bufs[R % 2],
-                                      c->partial + (size_t)tnext * pstride, k.wg, c->st);
-        CKL();
-      }
+      launch_contract_build<TS, TC>(L, msl, R, tnext + 1, LT[tnext], lev, prev, bufs[R % 2],
+                                    c->partial + (size_t)tnext * pstride, k.wg, kc == 4, c->st);
+      CKL();
       c->contract_launches += 1;
+      c->kind_launches[WLG_L_CHAIN_BUILD] += 1;
       tnext++;
     }
+    if (record(c, &eb)) return -1;
     if (!blocks.empty()) {
       for (size_t q = 0; q < blocks.size(); q++) {
         const DepthBlock& b = blocks[q];
@@ -627,17 +681,22 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
         CKL();
       }
       c->contract_launches += (long)blocks.size();
+      c->kind_launches[WLG_L_FUSED] += (long)blocks.size();
     }
+    if (record(c, &ef)) return -1;
     for (int t = tnext; t < p->mwlt; t++) {
       launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride, kc,
                               k.wg, c->st);
       CKL();
     }
     c->contract_launches += p->mwlt - tnext;
+    c->kind_launches[WLG_L_CHAIN] += p->mwlt - tnext;
+    if (record(c, &ec)) return -1;
     // W of this R (normalisation and wl_idx store, wloop.c:535, :584-597)
     launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, r, contract_slots(L, k.kernel, k.wg),
                         c->partial, norm, c->wl_dev, c->st);
     CKL();
+    c->kind_launches[WLG_L_FINISH] += 1;
     if (record(c, &e2)) return -1;
   }
   // the transporters of the last R stay in the arena (pus[msl+sl] of the
@@ -659,7 +718,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   const std::vector<DepthBlock> blocks = fused_blocks(k, prec, p->mwlt);
   int nb64 = 0;
   for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
-  Layout o = wloop_layout(L, p, prec, nb64);
+  Layout o = wloop_layout(L, p, prec, nb64, c->tune[WLG_TUNE_LEAN] == 1, blocks, k.kernel == 3);
   if (scratch(c, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
   if (ensure(c, (char**)&c->wl_dev, &c->wl_bytes, wls * sizeof(double))) return -1;
@@ -695,9 +754,15 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     launch_d2f((long)9 * L.V, U, lt32, c->st);  // L_1 = U_0 in FP32
     CKL();
     lts[0] = lt32;
-    int m64 = 0;
+    int m64 = 0, m32 = 0, n32 = 0;
+    for (const DepthBlock& b : blocks) n32 += !b.fp64 && b.Ta > 1;
     for (int T = 2; T <= tmax; T++) {
       float2* out32 = lt32 + (size_t)(T - 1) * 9 * L.V;
+      if (o.lean_lt && T < kTimeFp64From) {  // [L_1][block starts][two in flight]
+        bool start = false;
+        for (const DepthBlock& b : blocks) start = start || (!b.fp64 && b.Ta == T);
+        out32 = lt32 + (size_t)(start ? 1 + m32++ : 1 + n32 + T % 2) * 9 * L.V;
+      }
       if (T < kTimeFp64From) {
         launch_time_transporter<float, float, float, float>(L, lt32, 0, lts[T - 2], out32, nullptr,
                                                             c->st);
@@ -733,8 +798,15 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     std::vector<const double2*> lts(p->mwlt, nullptr);
     lts[0] = U;  // slot 0 of the gauge field
     double2* ltb = (double2*)(A + o.lt);
+    int m64 = 0, n64 = 0;
+    for (const DepthBlock& b : blocks) n64 += b.Ta > 1;
     for (int T = 2; T <= tmax; T++) {
       double2* out = ltb + (size_t)(T - 2) * 9 * L.V;
+      if (o.lean_lt) {  // [block starts][two in flight]
+        bool start = false;
+        for (const DepthBlock& b : blocks) start = start || b.Ta == T;
+        out = ltb + (size_t)(start ? m64++ : n64 + T % 2) * 9 * L.V;
+      }
       launch_time_transporter<double, double, double, double>(L, U, 0, lts[T - 2], out, nullptr,
                                                               c->st);
       CKL();
@@ -767,13 +839,19 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   }
   CK(hipEventElapsedTime(&ms, e_lev, e_tt));
   c->t_ms[WLG_T_TIME_TR] += ms;
-  for (size_t q = ev_before_rt; q + 2 < ev_after_rt; q += 3) {
+  static const int kinds[4] = {WLG_L_CHAIN_BUILD, WLG_L_FUSED, WLG_L_CHAIN, WLG_L_FINISH};
+  for (size_t q = ev_before_rt; q + 5 < ev_after_rt; q += 6) {  // one R (see run_transport_contract)
     float a = 0.f, b = 0.f;
     CK(hipEventElapsedTime(&a, c->ev[q], c->ev[q + 1]));
-    CK(hipEventElapsedTime(&b, c->ev[q + 1], c->ev[q + 2]));
+    CK(hipEventElapsedTime(&b, c->ev[q + 1], c->ev[q + 5]));
     c->t_ms[WLG_T_SPACE_TR] += a;
     c->t_ms[WLG_T_CONTRACT] += b;
     c->contract_ms += b;
+    for (int m = 0; m < 4; m++) {
+      float d = 0.f;
+      CK(hipEventElapsedTime(&d, c->ev[q + 1 + m], c->ev[q + 2 + m]));
+      c->kind_ms[kinds[m]] += d;
+    }
   }
   CK(hipEventElapsedTime(&ms, c->ev[ev_after_rt - 1], e_end));
   c->t_ms[WLG_T_FINISH] += ms;
@@ -847,6 +925,10 @@ extern "C" void wlg_timers_reset(wlg_ctx* c) {
   for (int i = 0; i < WLG_T_COUNT; i++) c->t_ms[i] = 0.0;
   c->contract_ms = 0.0;
   c->contract_launches = 0;
+  for (int m = 0; m < WLG_L_COUNT; m++) {
+    c->kind_launches[m] = 0;
+    c->kind_ms[m] = 0.0;
+  }
 }
 
 extern "C" double wlg_timer_ms(wlg_ctx* c, int stage) {
@@ -859,10 +941,17 @@ extern "C" int wlg_contract_stats(wlg_ctx* c, long* launches, double* ms) {
   return 0;
 }
 
+extern "C" int wlg_launch_stats(wlg_ctx* c, int kind, long* launches, double* ms) {
+  if (kind < 0 || kind >= WLG_L_COUNT) return set_err("unknown launch kind %d", kind);
+  *launches = c->kind_launches[kind];
+  *ms = c->kind_ms[kind];
+  return 0;
+}
+
 extern "C" int wlg_levels(wlg_ctx* c, const wlg_parms* p, int prec) {
   CK(hipSetDevice(c->device));
   if (check_parms(c, p)) return -1;
-  Layout o = wloop_layout(c->L, p, prec, 0);
+  Layout o = wloop_layout(c->L, p, prec, 0, c->tune[WLG_TUNE_LEAN] == 1, {}, false);
   if (scratch(c, o.total)) return -1;
   const double2* lev[5];
   const float2* lev32[5];
@@ -932,8 +1021,12 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if ((param == WLG_TUNE_FP64_CONTRACT_STARTS || param == WLG_TUNE_MIXED_CONTRACT_STARTS) &&
       value != 0 && value != 4 && value != 8)
     return set_err("start slices per workgroup must be 0 (default), 4 or 8");
-  if (param == WLG_TUNE_APE_FORM && value > 2)
-    return set_err("APE form must be 0 (default), 1 (one kernel) or 2 (staples, then projection)");
+  if (param == WLG_TUNE_APE_FORM && value > 4)
+    return set_err("APE form must be 0 (default), 1 (one thread per link), 2 (staples per link, "
+                   "then projection), 3 (one thread per site) or 4 (staples per site, then "
+                   "projection)");
+  if (param == WLG_TUNE_LEAN && value > 1)
+    return set_err("lean layout must be 0 (default layout) or 1 (least device memory)");
   if ((param == WLG_TUNE_FP64_CONTRACT_WG || param == WLG_TUNE_MIXED_CONTRACT_WG) && value > 65536)
     return set_err("workgroup parameter must be 0 (default) or 1..65536");
   c->tune[param] = value;
@@ -953,8 +1046,10 @@ static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
   if (!c->Uh) CK(hipMalloc((void**)&c->Uh, 4 * slot64(L)));
   c->cur_msl = p->msl;
   const Contract k = contract_choice(c, prec);
-  const int nb64 = k.kernel == 3 ? depth_fp64_starts(prec, p->mwlt, k.depth) : 0;
-  Layout o = wloop_layout(L, p, prec, nb64);
+  const std::vector<DepthBlock> blocks = fused_blocks(k, prec, p->mwlt);
+  int nb64 = 0;
+  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
+  Layout o = wloop_layout(L, p, prec, nb64, c->tune[WLG_TUNE_LEAN] == 1, blocks, k.kernel == 3);
   size_t hyp = 24 * slot64(L);
   if (scratch(c, o.total > hyp ? o.total : hyp)) return -1;
   size_t wls = (size_t)p->msl * p->msl * p->mwlt * p->mwlr * L.n[0];

@@ -17,11 +17,15 @@ void launch_hyp3(const Lat& L, const v2<TS>* F, v2<TS>* D, v2<TS>* out, double a
 template <class TC, class TS>
 void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
                 hipStream_t st);
-// APE step as a staple-sum kernel (sums into S, 3 slots of TC) and a
-// lincomb/projection kernel
+// APE step with one thread per site for its three spatial links
+template <class TC, class TS>
+void launch_ape_site(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
+                     hipStream_t st);
+// APE step as a staple-sum kernel (sums into S, 3 slots of TC; per link, or
+// per site when site != 0) and a lincomb/projection kernel
 template <class TC, class TS>
 void launch_ape_split(const Lat& L, const v2<TS>* in, v2<TC>* S, v2<TS>* out, double alpha, int proj,
-                      int piter, hipStream_t st);
+                      int piter, int site, hipStream_t st);
 void launch_ape4(const Lat& L, const double2* in, double2* out, double alpha, int proj, int piter,
                  hipStream_t st);
 void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2* v, int mu, int nu,
@@ -52,13 +56,13 @@ template <class TS, class TC>
 void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                      const v2<TS>* const* S, double* partial, int kernel, int wg,
                      hipStream_t st);
-// one-lane chain form for one (R,T) that also forms the spatial transporters
-// of R (out: msl 3-slot fields, S_R = F S_{R-1}(x+k)), which it contracts;
-// msl <= 4
+// chain form for one (R,T) that also forms the spatial transporters of R
+// (out: msl 3-slot fields, S_R = F S_{R-1}(x+k)), which it contracts; the
+// two-lane form if two_lane and msl is 2 or 4, else the one-lane form; msl <= 4
 template <class TS, class TC>
 void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                            const v2<TS>* const* F, const v2<TS>* const* prev, v2<TS>* out,
-                           double* partial, int wg, hipStream_t st);
+                           double* partial, int wg, int two_lane, hipStream_t st);
 // fused time-slice form: T = Ta .. Ta+d-1 of one R, W start slices per
 // workgroup (one wave each), nslot workgroups per start group; partial points
 // at the block of T = Ta, pstride apart per T

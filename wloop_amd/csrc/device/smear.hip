@@ -36,18 +36,18 @@ __device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, site_t s, co
   M3<TS> a, b, c;
   const site_t xpn = fwd(L, s, x, nu, 1);
   const site_t xpm = fwd(L, s, x, mu, 1);
-  load(b, B, L.V, lm, xpn);
-  load(c, A, L.V, ln, xpm);
-  load(a, A, L.V, ln, s);
+  load_u(b, B, L.V, lm, xpn);
+  load_u(c, A, L.V, ln, xpm);
+  load_u(a, A, L.V, ln, s);
   M3<TC> t = mul_dag(convert<TC>(b), convert<TC>(c));
   mul_acc(v, convert<TC>(a), t);
   const site_t xmn = bwd(L, s, x, nu);
   int y[4] = {x[0], x[1], x[2], x[3]};
   y[nu] = (x[nu] == 0) ? L.n[nu] - 1 : x[nu] - 1;
   const site_t xmnpm = fwd(L, xmn, y, mu, 1);
-  load(b, B, L.V, lm, xmn);
-  load(c, A, L.V, ln, xmnpm);
-  load(a, A, L.V, ln, xmn);
+  load_u(b, B, L.V, lm, xmn);
+  load_u(c, A, L.V, ln, xmnpm);
+  load_u(a, A, L.V, ln, xmn);
   t = mul(convert<TC>(b), convert<TC>(c));
   dag_mul_acc(v, convert<TC>(a), t);
 }
@@ -58,7 +58,7 @@ __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __
                                                v2<TS>* __restrict__ out, int lo, long V, site_t s,
                                                double alpha, double beta, int proj, int piter) {
   M3<TS> u;
-  load(u, U, V, lu, s);
+  load_u(u, U, V, lu, s);
   M3<double> w;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
@@ -66,7 +66,7 @@ __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __
     w.im[e] = alpha * (double)u.im[e] + beta * (double)S.im[e];
   }
   if (proj) project(w, piter);
-  store(convert<TS>(w), out, V, lo, s);
+  store_u(convert<TS>(w), out, V, lo, s);
 }
 
 __device__ __forceinline__ int part4(int a, int b) {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>
   if (open_bc && mu == 0 && x[0] == L.n[0] - 1) {
     M3<TS> z;
     set_zero(z);
-    store(z, out, L.V, 0, s);  // set_bc: U_0(x0 = N0-1) = 0
+    store_u(z, out, L.V, 0, s);  // set_bc: U_0(x0 = N0-1) = 0
   }
 }
 
@@ -217,6 +217,102 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2
   decorate_store<TC, TS>(S, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
 }
 
+// ---- APE step, one thread per site -------------------------------------------
+// The three spatial links of a site are smeared by one thread.  The up
+// staples of links mu and eta in the plane (mu,eta), mu < eta, share one
+// product:
+//     up staple of mu  = U_eta(x) Z,      Z = U_mu(x+eta) U_eta(x+mu)^dag
+//     up staple of eta = U_mu(x)  Z^dag
+// so each plane costs 1 + 2 + 2x2 = 7 products instead of 8 (21 per site),
+// and every link the staples of a site read is loaded by one thread.  The
+// planes are taken in the order (1,2), (1,3), (2,3); per link the staples are
+// then summed with eta ascending, up before down, as in ape_step.
+
+// v += U_eta(x-eta)^dag U_mu(x-eta) U_eta(x-eta+mu)
+template <class TC, class TS>
+__device__ __forceinline__ void down_staple(M3<TC>& v, const Lat& L, site_t s, const int x[4],
+                                            const v2<TS>* __restrict__ in, int mu, int eta) {
+  M3<TS> a, b, c;
+  const site_t xme = bwd(L, s, x, eta);
+  int y[4] = {x[0], x[1], x[2], x[3]};
+  y[eta] = (x[eta] == 0) ? L.n[eta] - 1 : x[eta] - 1;
+  load_u(b, in, L.V, mu - 1, xme);
+  load_u(c, in, L.V, eta - 1, fwd(L, xme, y, mu, 1));
+  load_u(a, in, L.V, eta - 1, xme);
+  const M3<TC> t = mul(convert<TC>(b), convert<TC>(c));
+  dag_mul_acc(v, convert<TC>(a), t);
+}
+
+// the up and down staples of links mu and eta (mu < eta) in their plane
+template <class TC, class TS>
+__device__ __forceinline__ void plane_staples(M3<TC>& Smu, M3<TC>& Seta, const Lat& L, site_t s,
+                                              const int x[4], const v2<TS>* __restrict__ in, int mu,
+                                              int eta) {
+  M3<TC> z;
+  {
+    M3<TS> b, c;
+    const site_t s1 = after(s, Seta);
+    load_u(b, in, L.V, mu - 1, fwd(L, s1, x, eta, 1));
+    load_u(c, in, L.V, eta - 1, fwd(L, s1, x, mu, 1));
+    z = mul_dag(convert<TC>(b), convert<TC>(c));
+  }
+  M3<TS> a;
+  load_u(a, in, L.V, eta - 1, s);
+  mul_acc(Smu, convert<TC>(a), z);
+  load_u(a, in, L.V, mu - 1, s);
+  mul_dag_acc(Seta, convert<TC>(a), z);
+  down_staple<TC, TS>(Smu, L, after(s, Seta), x, in, mu, eta);
+  down_staple<TC, TS>(Seta, L, after(s, Smu), x, in, eta, mu);
+}
+
+// the staple sums S[mu-1] of the three spatial links of a site
+template <class TC, class TS>
+__device__ __forceinline__ void site_staples(M3<TC> S[3], const Lat& L, site_t s, const int x[4],
+                                             const v2<TS>* __restrict__ in) {
+  set_zero(S[0]);
+  set_zero(S[1]);
+  set_zero(S[2]);
+  plane_staples<TC, TS>(S[0], S[1], L, s, x, in, 1, 2);
+  plane_staples<TC, TS>(S[0], S[2], L, s, x, in, 1, 3);
+  plane_staples<TC, TS>(S[1], S[2], L, s, x, in, 2, 3);
+}
+
+__device__ __forceinline__ site_t block_site() {
+  return (site_t)xcd_order(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+template <class TC, class TS>
+__global__ __launch_bounds__(256, sizeof(TS) == 4 ? 3 : 2) void ape_site(Lat L, const v2<TS>* __restrict__ in,
+                                                                      v2<TS>* __restrict__ out,
+                                                                      double alpha, int proj,
+                                                                      int piter) {
+  const site_t s = block_site();
+  if (s >= L.V) return;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> S[3];
+  site_staples<TC, TS>(S, L, s, x, in);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    __builtin_amdgcn_sched_barrier(0);  // one link's projection at a time
+    decorate_store<TC, TS>(S[k], in, k, out, k, L.V, s, 1.0, alpha, proj, piter);
+  }
+}
+
+// the staple sums of ape_site, stored (3 slots of TC) for ape_project
+template <class TC, class TS>
+__global__ __launch_bounds__(256, sizeof(TC) == 4 ? 3 : 2) void ape_site_staples(Lat L, const v2<TS>* __restrict__ in,
+                                                                              v2<TC>* __restrict__ S) {
+  const site_t s = block_site();
+  if (s >= L.V) return;
+  int x[4];
+  coords(L, s, x);
+  M3<TC> v[3];
+  site_staples<TC, TS>(v, L, s, x, in);
+#pragma unroll
+  for (int k = 0; k < 3; k++) store_u(v[k], S, L.V, k, s);
+}
+
 // The APE step in two kernels: the staple sums of every spatial link (in TC,
 // stored in TC), then the FP64 lincomb and projection.  The first is light in
 // registers and bound by link-load latency, the second is dense FP64 work on
@@ -237,7 +333,7 @@ __global__ __launch_bounds__(256, sizeof(TC) == 4 ? 4 : 2) void ape_staples(Lat 
     if (eta == mu) continue;
     staple_add<TC, TS>(v, L, s, x, in, eta - 1, in, mu - 1, mu, eta);
   }
-  store(v, S, L.V, mu - 1, s);
+  store_u(v, S, L.V, mu - 1, s);
 }
 
 template <class TC, class TS>
@@ -250,7 +346,7 @@ __global__ __launch_bounds__(256, 3) void ape_project(Lat L, const v2<TS>* __res
   if (s >= L.V) return;
   int mu = 1 + y;
   M3<TC> v;
-  load(v, S, L.V, mu - 1, s);
+  load_u(v, S, L.V, mu - 1, s);
   decorate_store<TC, TS>(v, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
 }
 
@@ -283,9 +379,9 @@ __global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* _
   int x[4];
   coords(L, s, x);
   M3<double> S;
-  load(S, v, L.V, mu, s);
+  load_u(S, v, L.V, mu, s);
   staple_add<double, double>(S, L, s, x, A, nu, B, mu, mu, nu);
-  store(S, v, L.V, mu, s);
+  store_u(S, v, L.V, mu, s);
 }
 
 // ---- launchers --------------------------------------------------------------
@@ -325,9 +421,19 @@ void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int p
 }
 
 template <class TC, class TS>
+void launch_ape_site(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int proj, int piter,
+                     hipStream_t st) {
+  hipLaunchKernelGGL((ape_site<TC, TS>), grid_for(L.V, 1), dim3(256), 0, st, L, in, out, alpha, proj,
+                     piter);
+}
+
+template <class TC, class TS>
 void launch_ape_split(const Lat& L, const v2<TS>* in, v2<TC>* S, v2<TS>* out, double alpha, int proj,
-                      int piter, hipStream_t st) {
-  hipLaunchKernelGGL((ape_staples<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, S);
+                      int piter, int site, hipStream_t st) {
+  if (site)
+    hipLaunchKernelGGL((ape_site_staples<TC, TS>), grid_for(L.V, 1), dim3(256), 0, st, L, in, S);
+  else
+    hipLaunchKernelGGL((ape_staples<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, S);
   hipLaunchKernelGGL((ape_project<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, S, out, alpha,
                      proj, piter);
 }
@@ -358,9 +464,13 @@ template void launch_ape<float, double>(const Lat&, const double2*, double2*, do
 template void launch_ape<float, float>(const Lat&, const float2*, float2*, double, int, int,
                                        hipStream_t);
 template void launch_ape_split<double, double>(const Lat&, const double2*, double2*, double2*, double,
-                                               int, int, hipStream_t);
+                                               int, int, int, hipStream_t);
 template void launch_ape_split<float, float>(const Lat&, const float2*, float2*, float2*, double, int,
-                                             int, hipStream_t);
+                                             int, int, hipStream_t);
+template void launch_ape_site<double, double>(const Lat&, const double2*, double2*, double, int, int,
+                                              hipStream_t);
+template void launch_ape_site<float, float>(const Lat&, const float2*, float2*, double, int, int,
+                                            hipStream_t);
 template void launch_hyp3<float, float>(const Lat&, const float2*, float2*, float2*, double, double,
                                         int, int, hipStream_t);
 

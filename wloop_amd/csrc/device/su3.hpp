@@ -58,6 +58,61 @@ __device__ __forceinline__ void load(M3<R>& m, const v2<R>* __restrict__ f, long
   }
 }
 
+// Loads and stores through a base pointer every lane of the wavefront holds
+// the same value of: the base goes to scalar registers (readfirstlane) as a
+// global-memory pointer, so each access is the scalar-base form (base +
+// 32-bit lane offset) instead of a 64-bit vector add per access.  Only for
+// bases that are uniform across the wavefront.
+template <class T> using gptr = __attribute__((address_space(1))) T*;
+
+template <class T> __device__ __forceinline__ gptr<T> uniform_ptr(const T* p) {
+  const uint64_t u = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+  return (gptr<T>)(((uint64_t)hi << 32) | lo);
+}
+
+template <class R> __device__ __forceinline__ v2<R> ld_u(const v2<R>* base, site_t s) {
+  typedef R pair __attribute__((ext_vector_type(2)));
+  gptr<const char> b = (gptr<const char>)uniform_ptr(base);
+  const pair t = *(gptr<const pair>)(b + (uint32_t)(s * (uint32_t)sizeof(pair)));
+  v2<R> v;
+  v.x = t.x;
+  v.y = t.y;
+  return v;
+}
+
+template <class R> __device__ __forceinline__ void st_u(v2<R>* base, site_t s, v2<R> v) {
+  typedef R pair __attribute__((ext_vector_type(2)));
+  gptr<char> b = (gptr<char>)uniform_ptr(base);
+  pair t;
+  t.x = v.x;
+  t.y = v.y;
+  *(gptr<pair>)(b + (uint32_t)(s * (uint32_t)sizeof(pair))) = t;
+}
+
+// load / store of slot l, the same slot on every lane of the wavefront
+template <class R>
+__device__ __forceinline__ void load_u(M3<R>& m, const v2<R>* __restrict__ f, long V, int l, site_t s) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    v2<R> v = ld_u<R>(f + (long)(9 * l + e) * V, s);
+    m.re[e] = v.x;
+    m.im[e] = v.y;
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void store_u(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, site_t s) {
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    v2<R> v;
+    v.x = m.re[e];
+    v.y = m.im[e];
+    st_u<R>(f + (long)(9 * l + e) * V, s, v);
+  }
+}
+
 template <class R>
 __device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, site_t s) {
 #pragma unroll
@@ -67,6 +122,21 @@ __device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, lon
     v.y = m.im[e];
     *at(f + (long)(9 * l + e) * V, s) = v;
   }
+}
+
+// s, made to depend on v: loads whose addresses are computed from the result
+// cannot be issued before v is formed.  Where a thread works through several
+// links in turn, this keeps the compiler from issuing the loads of all of them
+// at once, which spills the live matrices.  (An empty asm statement; it moves
+// no data.)
+template <class R> __device__ __forceinline__ site_t after(site_t s, const M3<R>& v) {
+  asm volatile(""
+               : "+v"(s)
+               : "v"(v.re[0]), "v"(v.re[1]), "v"(v.re[2]), "v"(v.re[3]), "v"(v.re[4]),
+                 "v"(v.re[5]), "v"(v.re[6]), "v"(v.re[7]), "v"(v.re[8]), "v"(v.im[0]),
+                 "v"(v.im[1]), "v"(v.im[2]), "v"(v.im[3]), "v"(v.im[4]), "v"(v.im[5]),
+                 "v"(v.im[6]), "v"(v.im[7]), "v"(v.im[8]));
+  return s;
 }
 
 // ---- products (element order as the textbook sum over j = 0,1,2) ----------
@@ -190,6 +260,26 @@ __device__ __forceinline__ void dag_mul_acc(M3<R>& v, const M3<R>& a, const M3<R
     }
 }
 
+// v += a * b^dag
+template <class R>
+__device__ __forceinline__ void mul_dag_acc(M3<R>& v, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R re = v.re[3 * i + k], im = v.im[3 * i + k];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        re = fmadd(a.re[3 * i + j], b.re[3 * k + j], re);
+        im = fmadd(a.im[3 * i + j], b.re[3 * k + j], im);
+        re = fmadd(a.im[3 * i + j], b.im[3 * k + j], re);
+        im = fmadd(-a.re[3 * i + j], b.im[3 * k + j], im);
+      }
+      v.re[3 * i + k] = re;
+      v.im[3 * i + k] = im;
+    }
+}
+
 template <class R> __device__ __forceinline__ void acc(M3<R>& v, const M3<R>& a) {
 #pragma unroll
   for (int e = 0; e < 9; e++) { v.re[e] += a.re[e]; v.im[e] += a.im[e]; }
@@ -263,40 +353,79 @@ __device__ __forceinline__ void project(M3<double>& u, int iter) {
 #pragma unroll
   for (int e = 0; e < 9; e++) { u.re[e] *= d; u.im[e] *= d; }
   for (int it = 0; it < iter; it++) {
-    // x = (3 - u^dag u)/2, Hermitian: the 6 independent entries of u^dag u
-    M3<double> x;
+    // x = (3 - u^dag u)/2 is Hermitian: real diagonal xd[a], upper entries
+    // xr/xi[ab] (ab = 01, 02, 12), x_ba = conj(x_ab)
+    double xd[3], xr[3], xi[3];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < 3; a++) {
+      double re = u.re[a] * u.re[a];
+      re = fmadd(u.im[a], u.im[a], re);
+      re = fmadd(u.re[3 + a], u.re[3 + a], re);
+      re = fmadd(u.im[3 + a], u.im[3 + a], re);
+      re = fmadd(u.re[6 + a], u.re[6 + a], re);
+      re = fmadd(u.im[6 + a], u.im[6 + a], re);
+      xd[a] = fmadd(-0.5, re, 1.5);
+    }
 #pragma unroll
-      for (int b = a; b < 3; b++) {
-        double re = u.re[a] * u.re[b];
-        re = fmadd(u.im[a], u.im[b], re);
-        re = fmadd(u.re[3 + a], u.re[3 + b], re);
-        re = fmadd(u.im[3 + a], u.im[3 + b], re);
-        re = fmadd(u.re[6 + a], u.re[6 + b], re);
-        re = fmadd(u.im[6 + a], u.im[6 + b], re);
-        if (a == b) {
-          x.re[3 * a + a] = fmadd(-0.5, re, 1.5);
-          x.im[3 * a + a] = 0.0;
-        } else {
-          double im = u.re[a] * u.im[b];
-          im = fmadd(-u.im[a], u.re[b], im);
-          im = fmadd(u.re[3 + a], u.im[3 + b], im);
-          im = fmadd(-u.im[3 + a], u.re[3 + b], im);
-          im = fmadd(u.re[6 + a], u.im[6 + b], im);
-          im = fmadd(-u.im[6 + a], u.re[6 + b], im);
-          x.re[3 * a + b] = -0.5 * re;
-          x.im[3 * a + b] = -0.5 * im;
-          x.re[3 * b + a] = -0.5 * re;
-          x.im[3 * b + a] = 0.5 * im;
+    for (int q = 0; q < 3; q++) {
+      const int a = q == 2 ? 1 : 0, b = q == 0 ? 1 : 2;
+      double re = u.re[a] * u.re[b];
+      re = fmadd(u.im[a], u.im[b], re);
+      re = fmadd(u.re[3 + a], u.re[3 + b], re);
+      re = fmadd(u.im[3 + a], u.im[3 + b], re);
+      re = fmadd(u.re[6 + a], u.re[6 + b], re);
+      re = fmadd(u.im[6 + a], u.im[6 + b], re);
+      double im = u.re[a] * u.im[b];
+      im = fmadd(-u.im[a], u.re[b], im);
+      im = fmadd(u.re[3 + a], u.im[3 + b], im);
+      im = fmadd(-u.im[3 + a], u.re[3 + b], im);
+      im = fmadd(u.re[6 + a], u.im[6 + b], im);
+      im = fmadd(-u.im[6 + a], u.re[6 + b], im);
+      xr[q] = -0.5 * re;
+      xi[q] = -0.5 * im;
+    }
+    // X = u x, element (i,k) = sum_j u_ij x_jk in the order j = 0,1,2, the
+    // real diagonal entry of x taken as a real factor
+    M3<double> X;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const double ur = u.re[3 * i + j], ui = u.im[3 * i + j];
+          if (j == k) {
+            if (j == 0) {
+              re = ur * xd[k];
+              im = ui * xd[k];
+            } else {
+              re = fmadd(ur, xd[k], re);
+              im = fmadd(ui, xd[k], im);
+            }
+          } else {
+            // x_jk: upper entry if j < k, else the conjugate of x_kj
+            const int q = (j + k == 1) ? 0 : ((j + k == 2) ? 1 : 2);
+            const double br = xr[q], bi = j < k ? xi[q] : -xi[q];
+            if (j == 0) {
+              re = ur * br;
+              im = ur * bi;
+            } else {
+              re = fmadd(ur, br, re);
+              im = fmadd(ur, bi, im);
+            }
+            re = fmadd(-ui, bi, re);
+            im = fmadd(ui, br, im);
+          }
         }
+        X.re[3 * i + k] = re;
+        X.im[3 * i + k] = im;
       }
-    x = mul(u, x);
-    const double zi = det_im(x) * (-1.0 / 3.0);
+    const double zi = det_im(X) * (-1.0 / 3.0);
 #pragma unroll
     for (int e = 0; e < 9; e++) {
-      u.re[e] = fmadd(-zi, x.im[e], x.re[e]);
-      u.im[e] = fmadd(zi, x.re[e], x.im[e]);
+      u.re[e] = fmadd(-zi, X.im[e], X.re[e]);
+      u.im[e] = fmadd(zi, X.re[e], X.im[e]);
     }
   }
 }

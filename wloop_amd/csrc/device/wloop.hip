@@ -432,10 +432,10 @@ __device__ __forceinline__ float lane_xor1(float v) {
 // order at the end), so the waves of a workgroup do not wait for each other
 // at every step (4.0 against 4.2 ms per (R,T) in FP64 at 48^3x96).  Requires
 // msl = 2 or 4.
-template <int MSL, class TS, class TC, bool WACC>
+template <int MSL, class TS, class TC, bool WACC, bool BUILD>
 __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, int g,
                                                           const v2<TS>* __restrict__ LT,
-                                                          LevelPtrs<TS> S,
+                                                          LevelPtrs<TS> S, BuildPtrs<TS> B,
                                                           double* __restrict__ partial) {
   constexpr int MM = MSL * MSL;
   constexpr int MH = MSL / 2;
@@ -456,10 +456,28 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane & 1, pr = lane >> 1;
   TC* rw = redf + wave * 32 * 16;
-  // this lane's levels i = h MH + m
+  // this lane's levels i = h MH + m (BUILD: the level links F, the transporters
+  // S_{R-1} and the output S_R of those levels)
   const v2<TS>* sp[MH];
+  const v2<TS>* bf[MH];
+  const v2<TS>* bp[MH];
+  v2<TS>* bo[MH];
 #pragma unroll
-  for (int m = 0; m < MH; m++) sp[m] = level_ptr(S, h * MH + m);
+  for (int m = 0; m < MH; m++) {
+    sp[m] = level_ptr(S, h * MH + m);
+    if constexpr (BUILD) {
+      LevelPtrs<TS> f, q, o;
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        f.p[i] = B.f[i];
+        q.p[i] = B.prev[i];
+        o.p[i] = B.out[i];
+      }
+      bf[m] = level_ptr(f, h * MH + m);
+      bp[m] = level_ptr(q, h * MH + m);
+      bo[m] = const_cast<v2<TS>*>(level_ptr(o, h * MH + m));
+    }
+  }
 
   for (int q = threadIdx.x; q < NACC * N0 * MM; q += 256) acc[q] = 0.0;
   __syncthreads();
@@ -474,6 +492,19 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
     const long xs = live ? xs0 : 0;
     int x[4];
     coords(L, xs, x);
+    // level link m of this lane at site s (BUILD: formed as S_R(x,k) =
+    // F(x,k) S_{R-1}(x+k,k), the operands of space_transporter, and stored)
+    auto level = [&](M3<TS>& t, int m, long s) {
+      if constexpr (BUILD) {
+        M3<TS> f, q;
+        load(f, bf[m], L.V, k, s);
+        load(q, bp[m], L.V, k, fwd(L, s, x, k + 1, 1));
+        t = mul(f, q);
+        if (live) store(t, bo[m], L.V, k, s);
+      } else {
+        load(t, sp[m], L.V, k, s);
+      }
+    };
     M3<TC> D[MH], Sx[MH];
     {
       const long s0 = (long)c * slice + xs;
@@ -483,7 +514,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
       M3<TC> ac = convert<TC>(a), bc = convert<TC>(b);
 #pragma unroll
       for (int m = 0; m < MH; m++) {
-        load(t, sp[m], L.V, k, s0);
+        level(t, m, s0);
         D[m] = mul(dag_mul(ac, convert<TC>(t)), bc);
       }
     }
@@ -493,7 +524,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 #pragma unroll
     for (int m = 0; m < MH; m++) {
       M3<TS> t;
-      load(t, sp[m], L.V, k, (long)x1 * slice + xs);
+      level(t, m, (long)x1 * slice + xs);
       Sx[m] = convert<TC>(t);
     }
     for (int n = 0; n < len; n++) {
@@ -578,7 +609,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 #pragma unroll
         for (int m = 0; m < MH; m++) {
           M3<TS> t;
-          load(t, sp[m], L.V, k, (long)x1 * slice + xs);
+          level(t, m, (long)x1 * slice + xs);
           Sx[m] = convert<TC>(t);
         }
       }
@@ -850,32 +881,50 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
 }
 
 // W_ij(R,T;x0) of one R from the partials of its (R,T) launches, stored in
-// wl_idx order.  partial layout: [t][x0][slot][i*msl+j] (nwg slots per x0);
-// the slot sums are added in fixed order with Neumaier compensation, then
-// normalised.
+// wl_idx order.  partial layout: [t][x0][slot][i*msl+j] (nwg slots per x0).
+// One wave per (t, x0): lane l takes entry ij = l % mm of slot group
+// g = l / mm (G groups, the largest power of two with G mm <= 64) and sums the
+// slots g, g+G, ... with
+// Neumaier compensation (consecutive lanes read consecutive doubles); the G
+// compensated sums of an entry are then combined in a fixed butterfly order and
+// normalised.  Deterministic: the order depends only on nwg and msl.
+__device__ __forceinline__ void neumaier_add(double& s, double& c, double v) {
+  const double t = s + v;
+  c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+  s = t;
+}
+
 __global__ __launch_bounds__(256) void finish_wloop(int N0, int msl, int mwlt, int mwlr, int r,
                                                     int nwg, const double* __restrict__ partial,
                                                     double norm, double* __restrict__ wl) {
-  long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  long nout = (long)msl * msl * mwlt * N0;
-  if (id >= nout) return;
-  // id enumerates (i, j, t, x0) of this r
-  long r_ = id;
-  int i = (int)(r_ % msl); r_ /= msl;
-  int j = (int)(r_ % msl); r_ /= msl;
-  int t = (int)(r_ % mwlt); r_ /= mwlt;
-  int x0 = (int)r_;
   const int mm = msl * msl;
-  const double* p = partial + (((long)t * N0 + x0) * nwg) * mm + i * msl + j;
+  int G = 1;  // slot groups: the largest power of two with G * mm <= 64
+  while (2 * G * mm <= 64) G *= 2;
+  const int lane = threadIdx.x & 63;
+  const long tx = (long)blockIdx.x * 4 + (threadIdx.x >> 6);  // (t, x0) = t * N0 + x0
+  if (tx >= (long)mwlt * N0) return;
+  const int g = lane / mm, ij = lane % mm;
   double sum = 0.0, comp = 0.0;
-  for (int w = 0; w < nwg; w++) {
-    double v = p[(long)w * mm];
-    double t2 = sum + v;
-    comp += (fabs(sum) >= fabs(v)) ? (sum - t2) + v : (v - t2) + sum;
-    sum = t2;
+  if (g < G) {
+    const double* p = partial + tx * nwg * mm + ij;
+    for (int w = g; w < nwg; w += G) neumaier_add(sum, comp, p[(long)w * mm]);
   }
-  // wl_idx order: i + msl*(j + msl*(t + mwlt*(r + mwlr*x0)))
-  wl[i + (long)msl * (j + (long)msl * (t + (long)mwlt * (r + (long)mwlr * x0)))] = (sum + comp) / norm;
+  // combine the groups of an entry: partner lane + mm*h for h = G/2, G/4, ...
+  // (G is 2, 4, 4, 16, 64 for msl = 5..1)
+  for (int h = G / 2; h >= 1; h /= 2) {
+    const double os = __shfl_down(sum, h * mm, 64);
+    const double oc = __shfl_down(comp, h * mm, 64);
+    if (g < h) {
+      neumaier_add(sum, comp, os);
+      comp += oc;
+    }
+  }
+  if (g == 0) {
+    const int t = (int)(tx / N0), x0 = (int)(tx % N0);
+    const int i = ij / msl, j = ij % msl;
+    // wl_idx order: i + msl*(j + msl*(t + mwlt*(r + mwlr*x0)))
+    wl[i + (long)msl * (j + (long)msl * (t + (long)mwlt * (r + (long)mwlr * x0)))] = (sum + comp) / norm;
+  }
 }
 
 // ---- launchers ------------------------------------------------------------
@@ -945,10 +994,35 @@ static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
                        LT, sp, BuildPtrs<TS>{}, partial);
 }
 
+template <int MSL, class TS, class TC>
+static void chain2_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st,
+                          const BuildPtrs<TS>* build = nullptr) {
+  // per-wave accumulators where 4 of them fit into LDS (N0 <= 300 at msl 4)
+  const size_t rest = 2 * 4 * 16 * sizeof(double) + 4 * 32 * 16 * sizeof(TC);
+  const size_t acc1 = (size_t)L.n[0] * MSL * MSL * sizeof(double);
+  const BuildPtrs<TS> b = build ? *build : BuildPtrs<TS>{};
+  if (4 * acc1 + rest <= 163840) {
+    if (build)
+      hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true, true>), dim3(nwg), dim3(256), 4 * acc1 + rest,
+                         st, L, R, T, g, LT, sp, b, partial);
+    else
+      hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true, false>), dim3(nwg), dim3(256), 4 * acc1 + rest,
+                         st, L, R, T, g, LT, sp, b, partial);
+  } else {
+    if (build)
+      hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false, true>), dim3(nwg), dim3(256), acc1 + rest, st,
+                         L, R, T, g, LT, sp, b, partial);
+    else
+      hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false, false>), dim3(nwg), dim3(256), acc1 + rest, st,
+                         L, R, T, g, LT, sp, b, partial);
+  }
+}
+
 template <class TS, class TC>
 void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
                            const v2<TS>* const* F, const v2<TS>* const* prev, v2<TS>* out,
-                           double* partial, int wg, hipStream_t st) {
+                           double* partial, int wg, int two_lane, hipStream_t st) {
   LevelPtrs<TS> sp;
   BuildPtrs<TS> b;
   for (int i = 0; i < 5; i++) {
@@ -963,26 +1037,20 @@ void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT
     g = t;
     t = r;
   }
+  if (two_lane && msl == 2) {
+    chain2_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b);
+    return;
+  }
+  if (two_lane && msl == 4) {
+    chain2_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b);
+    return;
+  }
   switch (msl) {
     case 1: chain1_launch<1, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
     case 2: chain1_launch<2, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
     case 3: chain1_launch<3, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
     default: chain1_launch<4, TS, TC>(L, R, T, g, LT, sp, partial, wg, st, &b); break;
   }
-}
-
-template <int MSL, class TS, class TC>
-static void chain2_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
-                          const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st) {
-  // per-wave accumulators where 4 of them fit into LDS (N0 <= 300 at msl 4)
-  const size_t rest = 2 * 4 * 16 * sizeof(double) + 4 * 32 * 16 * sizeof(TC);
-  const size_t acc1 = (size_t)L.n[0] * MSL * MSL * sizeof(double);
-  if (4 * acc1 + rest <= 163840)
-    hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true>), dim3(nwg), dim3(256), 4 * acc1 + rest, st, L,
-                       R, T, g, LT, sp, partial);
-  else
-    hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false>), dim3(nwg), dim3(256), acc1 + rest, st, L, R,
-                       T, g, LT, sp, partial);
 }
 
 size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes) {
@@ -1072,9 +1140,9 @@ void launch_contract(const Lat& L, int msl, int R, int T, const v2<TS>* LT,
 
 void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st) {
-  long nout = (long)msl * msl * mwlt * N0;
-  hipLaunchKernelGGL(finish_wloop, grid_for(nout, 1), dim3(256), 0, st, N0, msl, mwlt, mwlr, r, nwg,
-                     partial, norm, wl);
+  const long waves = (long)mwlt * N0;  // one per (t, x0), four per workgroup
+  hipLaunchKernelGGL(finish_wloop, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, N0, msl, mwlt,
+                     mwlr, r, nwg, partial, norm, wl);
 }
 
 template void launch_time_transporter<double, double, double, double>(
@@ -1094,7 +1162,10 @@ template void launch_contract<float, float>(const Lat&, int, int, int, const flo
                                             const float2* const*, double*, int, int, hipStream_t);
 template void launch_contract_build<float, float>(const Lat&, int, int, int, const float2*,
                                                   const float2* const*, const float2* const*, float2*,
-                                                  double*, int, hipStream_t);
+                                                  double*, int, int, hipStream_t);
+template void launch_contract_build<double, double>(const Lat&, int, int, int, const double2*,
+                                                    const double2* const*, const double2* const*,
+                                                    double2*, double*, int, int, hipStream_t);
 template void launch_contract_depth<double, double>(const Lat&, int, int, int, int, int, int,
                                                     const double2*, const double2*,
                                                     const double2* const*, double*, long,

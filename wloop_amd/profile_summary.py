@@ -1,17 +1,28 @@
 """Summaries of rocprofv3 output for the Wilson-loop kernels.
 
-    python -m wloop_amd.profile_summary KT_DIR FETCH_DIR WRITE_DIR OUT_JSON
+    python -m wloop_amd.profile_summary KT_DIR FETCH_DIR WRITE_DIR SQ_DIRS OUT_JSON
     python -m wloop_amd.profile_summary --all KT_DIR FETCH_DIR WRITE_DIR
 
 KT_DIR holds a ``--kernel-trace --stats`` run (``*_kernel_stats.csv``),
 FETCH_DIR and WRITE_DIR ``--pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs of the
-same command (``*_counter_collection.csv``).  The byte counters are calibrated
-per dispatch against the FP64 -> FP32 copy kernel ``d2f`` (one complex number
-per thread: 16 B read and 8 B written per grid item).  Prints markdown tables
-(kernel times; HBM traffic per launch of the contraction kernels) and writes
-OUT_JSON with the calibrated bytes per launch keyed "<form>/<precision>", the
-keys bench.py reads for the ``roofline.traffic`` field.  With ``--all`` it prints
-the calibrated traffic and rate of every kernel of unfiltered counter runs.
+same command, SQ_DIRS a comma-separated list of ``--pmc`` runs of SQ counters
+(``*_counter_collection.csv``).  The byte counters are calibrated per dispatch
+against the FP64 -> FP32 copy kernel ``d2f`` (one complex number per thread:
+16 B read and 8 B written per grid item).
+
+Kernels are grouped by launch kind and precision ("ape_step/mixed",
+"fused/mixed", "chain/fp64", "chain_build/mixed", ...; see kernel_key).  Prints
+markdown tables (kernel time; HBM traffic per launch; SQ counters per launch
+with the VALU issue utilisation) and writes OUT_JSON with, per key, the
+calibrated HBM bytes per launch, the average launch time and the SQ counters
+per launch -- the figures bench.py reads for the ``traffic`` field of its
+roofline entries.  With ``--all`` it prints the calibrated traffic and rate of
+every kernel of unfiltered counter runs.
+
+VALU issue utilisation = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x launch time x
+2.4 GHz): a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles at
+full rate (FP32 without packed math and FP64 alike), 256 CUs x 4 SIMDs, at the
+chip's peak engine clock.
 """
 import collections
 import csv
@@ -21,8 +32,8 @@ import os
 import re
 import sys
 
-FORMS = (("contract_stream", 1), ("contract_chain1", 3), ("contract_chain2", 5), ("contract_chain", 2),
-         ("contract_depth", 4))
+SIMDS = 1024
+CLOCK_HZ = 2.4e9
 
 
 def _csv(d, suffix):
@@ -33,16 +44,41 @@ def _csv(d, suffix):
         return list(csv.DictReader(f))
 
 
+def _targs(name):
+    if "<" not in name:
+        return []
+    return [a.strip() for a in name[name.index("<") + 1:name.index(">")].split(",")]
+
+
 def kernel_key(name):
-    """'<form>/<precision>' of a contraction kernel symbol, else None; the
-    one-lane chain launch that also forms the spatial transporters is form '3b'"""
-    for stem, form in FORMS:
+    """'<kind>/<precision>' of a kernel symbol, else None.  Precision 'mixed'
+    for FP32 link storage, 'fp64' for FP64; the fused launches of the mixed
+    path that carry FP64 open loops (T >= 9) are 'fused64/mixed'; the chain
+    launch that also forms the spatial transporters is 'chain_build'."""
+    a = _targs(name)
+    if re.search(r"\bape_step<", name):
+        return "ape_step/" + ("mixed" if a[1] == "float" else "fp64")
+    m = re.search(r"\bhyp_level(\d)<", name)
+    if m:
+        return f"hyp{m.group(1)}/" + ("mixed" if a[0] == "float" else "fp64")
+    if re.search(r"\bcontract_depth<", name):
+        ts, tc = a[-2], a[-1]
+        if ts == "double":
+            return "fused/fp64"
+        return "fused/mixed" if tc == "float" else "fused64/mixed"
+    # template position of the link storage type of each per-(R,T) form
+    for stem, its in (("contract_chain1", 1), ("contract_chain2", 1), ("contract_stream", 1),
+                      ("contract_chain", 3)):
         if re.search(r"\b" + stem + r"<", name):
-            args = [a.strip() for a in name[name.index("<") + 1:name.index(">")].split(",")]
-            prec = "fp64" if args[-2] == "double" else "mixed"
-            if stem == "contract_chain1" and args[-1] == "true":
-                form = "3b"
-            return f"{form}/{prec}"
+            prec = "fp64" if a[its] == "double" else "mixed"
+            build = a[-1] == "true" and stem in ("contract_chain1", "contract_chain2")
+            return ("chain_build/" if build else "chain/") + prec
+    if re.search(r"\bspace_transporter<", name):
+        return "space_tr/" + ("mixed" if a[0] == "float" else "fp64")
+    if re.search(r"\btime_transporter<", name):
+        return "time_tr/" + ("mixed" if a[0] == "float" else "fp64")
+    if re.search(r"\bfinish_wloop\b", name):
+        return "finish/any"
     return None
 
 
@@ -51,7 +87,7 @@ def short(name, n=64):
     return name if len(name) <= n else name[:n - 1] + "~"
 
 
-def stats_table(kt_dir, top=14):
+def stats_table(kt_dir, top=16):
     rows = _csv(kt_dir, "_kernel_stats.csv")
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     out = ["| kernel | calls | avg ms | total ms | % |", "|---|---:|---:|---:|---:|"]
@@ -59,6 +95,16 @@ def stats_table(kt_dir, top=14):
         out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
                    f"{float(r['TotalDurationNs']) / 1e6:.1f} | {float(r['Percentage']):.1f} |")
     return "\n".join(out), rows
+
+
+def avg_ms_by_key(kt_dir):
+    agg = collections.defaultdict(lambda: [0.0, 0])
+    for r in _csv(kt_dir, "_kernel_stats.csv"):
+        k = kernel_key(r["Name"])
+        if k:
+            agg[k][0] += float(r["TotalDurationNs"]) / 1e6
+            agg[k][1] += int(r["Calls"])
+    return {k: t / n for k, (t, n) in agg.items() if n}
 
 
 def _per_dispatch(d, counter):
@@ -70,14 +116,18 @@ def _per_dispatch(d, counter):
     return out
 
 
-def traffic(fetch_dir, write_dir):
-    fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
-    write = _per_dispatch(write_dir, "WRITE_SIZE")
+def _calibration(fetch, write):
     cf = [16.0 * g / b for n, g, b in fetch if re.search(r"\bd2f\b", n) and b > 0]
     cw = [8.0 * g / b for n, g, b in write if re.search(r"\bd2f\b", n) and b > 0]
     if not cf or not cw:
         raise RuntimeError("no d2f dispatch to calibrate against")
-    kf, kw = sum(cf) / len(cf), sum(cw) / len(cw)
+    return sum(cf) / len(cf), sum(cw) / len(cw)
+
+
+def traffic(fetch_dir, write_dir):
+    fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = _per_dispatch(write_dir, "WRITE_SIZE")
+    kf, kw = _calibration(fetch, write)
     agg = collections.defaultdict(lambda: [[], []])
     names = {}
     for idx, rows in ((0, fetch), (1, write)):
@@ -88,11 +138,42 @@ def traffic(fetch_dir, write_dir):
                 names[key] = short(n)
     per_launch = {k: (sum(f) / len(f) if f else 0.0) + (sum(w) / len(w) if w else 0.0)
                   for k, (f, w) in agg.items()}
-    table = ["| kernel | launches | fetched GB/launch | written MB/launch |", "|---|---:|---:|---:|"]
+    table = ["| kernel (key) | launches | fetched GB/launch | written GB/launch |", "|---|---:|---:|---:|"]
     for k, (f, w) in sorted(agg.items()):
-        table.append(f"| `{names[k]}` ({k}) | {len(f)} | {sum(f) / len(f) / 1e9:.2f} | "
-                     f"{sum(w) / max(len(w), 1) / 1e6:.2f} |")
+        table.append(f"| `{names[k]}` ({k}) | {len(f)} | {sum(f) / max(len(f), 1) / 1e9:.3f} | "
+                     f"{sum(w) / max(len(w), 1) / 1e9:.3f} |")
     return kf, kw, per_launch, "\n".join(table)
+
+
+def sq_counters(dirs):
+    """{key: {counter: mean value per launch}} over the SQ counter runs"""
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in dirs:
+        for r in _csv(d, "_counter_collection.csv"):
+            k = kernel_key(r["Kernel_Name"])
+            if k:
+                agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+
+def sq_table(sq, avg_ms):
+    cols = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_FMA_F64",
+            "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS"]
+    out = ["| key | avg ms | " + " | ".join(c.replace("SQ_INSTS_", "").replace("SQ_", "") for c in cols) +
+           " | FMA share | VALU issue util |", "|---|" + "---:|" * (len(cols) + 3)]
+    derived = {}
+    for k in sorted(sq):
+        c = sq[k]
+        t = avg_ms.get(k)
+        valu = c.get("SQ_INSTS_VALU", 0.0)
+        fma = c.get("SQ_INSTS_VALU_FMA_F32", 0.0) + c.get("SQ_INSTS_VALU_FMA_F64", 0.0)
+        util = valu * 4.0 / (SIMDS * t * 1e-3 * CLOCK_HZ) if t else None
+        derived[k] = {"fma_share": fma / valu if valu else None, "valu_issue_util": util}
+        cells = [f"{c.get(x, 0.0):.4g}" for x in cols]
+        out.append(f"| {k} | {t:.3f} | " + " | ".join(cells) +
+                   f" | {derived[k]['fma_share'] or 0:.2f} | {util if util is not None else 0:.2f} |"
+                   if t else f"| {k} | -- | " + " | ".join(cells) + " | -- | -- |")
+    return "\n".join(out), derived
 
 
 def all_traffic(kt_dir, fetch_dir, write_dir):
@@ -100,11 +181,7 @@ def all_traffic(kt_dir, fetch_dir, write_dir):
     kernel of FETCH_DIR / WRITE_DIR runs made without a kernel filter"""
     fetch = _per_dispatch(fetch_dir, "FETCH_SIZE")
     write = _per_dispatch(write_dir, "WRITE_SIZE")
-    kf = [16.0 * g / b for n, g, b in fetch if re.search(r"\bd2f\b", n) and b > 0]
-    kw = [8.0 * g / b for n, g, b in write if re.search(r"\bd2f\b", n) and b > 0]
-    if not kf or not kw:
-        raise RuntimeError("no d2f dispatch to calibrate against")
-    kf, kw = sum(kf) / len(kf), sum(kw) / len(kw)
+    kf, kw = _calibration(fetch, write)
     avg = {short(r["Name"], 200): float(r["AverageNs"]) / 1e6 for r in _csv(kt_dir, "_kernel_stats.csv")}
     agg = collections.defaultdict(lambda: [[], []])
     for idx, rows in ((0, fetch), (1, write)):
@@ -126,18 +203,23 @@ def main(argv):
     if len(argv) == 5 and argv[1] == "--all":
         print(all_traffic(*argv[2:]))
         return 0
-    if len(argv) != 5:
+    if len(argv) != 6:
         print(__doc__, file=sys.stderr)
         return 2
-    kt_dir, fetch_dir, write_dir, out_json = argv[1:]
+    kt_dir, fetch_dir, write_dir, sq_dirs, out_json = argv[1:]
     st, _ = stats_table(kt_dir)
+    avg_ms = avg_ms_by_key(kt_dir)
     kf, kw, per_launch, tt = traffic(fetch_dir, write_dir)
+    sq = sq_counters([d for d in sq_dirs.split(",") if d])
+    sqt, derived = sq_table(sq, avg_ms)
     print("## Kernel time\n\n" + st + "\n")
     print(f"## HBM traffic per launch\n\nCalibration against `d2f`: FETCH_SIZE x {kf:.4f}, "
           f"WRITE_SIZE x {kw:.4f}.\n\n" + tt + "\n")
+    print("## SQ counters per launch\n\n" + sqt + "\n")
     with open(out_json, "w") as f:
-        json.dump({"fetch_calibration": kf, "write_calibration": kw, "bytes_per_launch": per_launch},
-                  f, indent=1)
+        json.dump({"fetch_calibration": kf, "write_calibration": kw, "avg_launch_ms": avg_ms,
+                   "bytes_per_launch": per_launch, "sq_per_launch": sq, "sq_derived": derived},
+                  f, indent=1, sort_keys=True)
     return 0
 
 

@@ -9,8 +9,10 @@ measurement (HYP, operator levels, transporters, W_ij(R,T;x0)) there.
   next request of that shape.
 * Admission by HBM footprint: a request starts only when the device bytes its
   context needs (wlg_footprint) fit in the budget next to the contexts that
-  already exist; idle contexts are evicted least-recently-used first; a
-  request that cannot fit even on an empty device is rejected.
+  already exist; idle contexts are evicted least-recently-used first.  A shape
+  whose default layout does not fit the budget even alone gets a context with
+  the lean layout (wlg_footprint_lean: fused contraction, time transporters
+  only at block starts), and only a request that fits neither is rejected.
 * Workers run requests concurrently, each context on its own HIP stream; the
   next request is the one of least predicted cost (with an ageing credit and
   a bonus for a resident context of its shape), or the oldest with --policy fifo.
@@ -51,6 +53,16 @@ def standard_request(rid, L, bc, precision, seed=None, eps=0.3, nss=NSS):
     h = L // 2
     p = W.make_parms(list(nss), h, h, alpha_ape=0.5, proj_iter=4, alpha_action=(1.0, 1.0, 0.5))
     return Request(rid, (2 * L, L, L, L), bc, rid if seed is None else seed, eps, p, precision)
+
+
+def context_layout(req, budget):
+    """(lean, bytes) of the context a request needs: the default layout if it
+    fits the budget by itself, else the lean layout (which may still not fit:
+    admission then rejects the request)"""
+    nbytes = W.footprint(req.dims, req.parms, req.precision)
+    if nbytes <= budget:
+        return False, nbytes
+    return True, W.footprint(req.dims, req.parms, req.precision, lean=True)
 
 
 class Admission:
@@ -205,11 +217,13 @@ class Server:
 
     def _serve(self, req):
         key = (req.dims, req.bc, req.precision)
-        nbytes = W.footprint(req.dims, req.parms, req.precision)
+        lean, nbytes = context_layout(req, self.adm.budget)
         lat, create = self.adm.acquire(key, nbytes, lambda h: h.close())
         if create:
             try:
                 lat = W.Lattice(req.dims, req.bc, self.device)
+                if lean:
+                    lat.tune(W.TUNE_LEAN, 1)
             except Exception:
                 self.adm.abandon(nbytes)
                 raise
