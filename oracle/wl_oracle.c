@@ -17,8 +17,11 @@
  *                     in numpy (hep-lat/0103029), HYP(a,0,0)=(1-a)APE4d
  *                     (devel/smear/check3.c:95-121), gauge covariance
  *   wo_hyp_spatial    HYP3d(a,0)=(1-a)APE3d (devel/smear/check3.c:136-182);
- *                     the alpha2!=0 wiring follows hyp_spatial_links.c
- *                     literally -- parity unpinned beyond that KAT
+ *                     alpha2!=0: closed form on an abelian constant-flux
+ *                     field with a distinct flux per plane, which holds for
+ *                     the wiring of hyp_spatial_links.c:187-234 (level-1
+ *                     links decorated in the staple's own plane) and fails
+ *                     for the textbook one
  *   wo_ape            unit field -> (1+4a)1 unprojected, gauge covariance,
  *                     brute-force numpy APE step
  *   wo_wloop          unit field -> W=1 (devel/wloop/check1.c), gauge

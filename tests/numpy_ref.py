@@ -151,3 +151,64 @@ def wilson_loops_walk(U, levels, mwlt, mwlr):
                         tr = np.real(np.trace(M, axis1=-2, axis2=-1))
                         W[:, R - 1, T - 1, i, j] += tr.sum(axis=(1, 2, 3))
     return W / (9.0 * N1 * N2 * N3)
+
+
+def abelian_flux_field(dims, n12, n13, n23):
+    """Spatial links U_k = diag(e^{i th_k}, e^{-i th_k}, 1) with a constant
+    plaquette angle F_kl in each spatial plane (th_1 = 0, th_2 = F12 x1,
+    th_3 = F13 x1 + F23 x2, F12 = 2 pi n12 / N1, F13 = 2 pi n13 / N1,
+    F23 = 2 pi n23 / N2, so the field is periodic); time links 1.
+    Returns the field [V][4][18] and {(k, l): F_kl}."""
+    f12, f13, f23 = (2 * np.pi * n12 / dims[1], 2 * np.pi * n13 / dims[1], 2 * np.pi * n23 / dims[2])
+    x = np.indices(dims)
+    th = {1: np.zeros(dims), 2: f12 * x[1], 3: f13 * x[1] + f23 * x[2]}
+    g = np.zeros(dims + (4, 3, 3), dtype=complex)
+    g[..., 0, :, :] = np.eye(3)
+    for k in (1, 2, 3):
+        g[..., k, 0, 0] = np.exp(1j * th[k])
+        g[..., k, 1, 1] = np.exp(-1j * th[k])
+        g[..., k, 2, 2] = 1.0
+    F = {(1, 2): f12, (1, 3): f13, (2, 3): f23}
+    F.update({(b, a): v for (a, b), v in list(F.items())})
+    return from_grid(g), F
+
+
+def hyp3_abelian_factors(F, a1, a2, wiring):
+    """3-d HYP without projection of an abelian constant-flux field: every
+    smeared link is U_k times a real diagonal factor, per colour charge
+    q = (1, -1, 0).  The staple sum of a plane with links scaled by a (nu) and
+    b (mu) is a^2 b U_mu 2 cos(q F); level 1 decorates U_k in the plane (k,l)
+    with g(F_kl) = (1 - a2) + a2 cos(q F_kl).  Level 2 takes, for the staple
+    of link mu in the plane (mu,nu),
+      wiring "reference": nu-links decorated in the plane (nu,mu) and mu-links
+        decorated in (mu,nu) (hyp_spatial_links.c:187-234, dlnks_idx :116-121);
+      wiring "textbook": nu-links decorated in (nu,rho) and mu-links in
+        (mu,rho), rho the third direction (hep-lat/0103029).
+    Returns {k: factor per colour (3,)}."""
+    q = np.array([1.0, -1.0, 0.0])
+
+    def g(f):
+        return (1 - a2) + a2 * np.cos(q * f)
+
+    out = {}
+    for mu in (1, 2, 3):
+        s = np.zeros(3)
+        for nu in (1, 2, 3):
+            if nu == mu:
+                continue
+            rho = 6 - mu - nu
+            if wiring == "reference":
+                a, b = g(F[(nu, mu)]), g(F[(mu, nu)])
+            else:
+                a, b = g(F[(nu, rho)]), g(F[(mu, rho)])
+            s += a * a * b * np.cos(q * F[(mu, nu)])
+        out[mu] = (1 - a1) + (a1 / 2) * s
+    return out
+
+
+def scale_links(u, dims, factors):
+    """U_k -> U_k diag(factors[k]) for k = 1, 2, 3"""
+    g = to_grid(u, dims).copy()
+    for k, f in factors.items():
+        g[..., k, :, :] = g[..., k, :, :] * f[None, :]
+    return from_grid(g)

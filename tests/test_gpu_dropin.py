@@ -231,3 +231,31 @@ def test_get_pus_after_wloop_sum(C, keep):
         got = O.to_complex(pus_level(C, pus, ipt, V, 2 + sl).reshape(-1, 1, 18)).reshape(sr[sl].shape)
         assert np.max(np.abs(got - sr[sl])) < 1e-13
     C.free_smear()
+
+
+@pytest.mark.parametrize("mu,nu", [(1, 2), (0, 3), (3, 0)])
+def test_staple_sum_openqcd_layout(C, mu, nu):
+    """staple_sum(u1, u2, mu, nu, vb, cbuf): vb(x,mu) += the staples with the
+    nu-links from u1 and the mu-links from u2 (staple_sum.c:53-87), fields in
+    openQCD's layout; vb's other directions are left as they are."""
+    dims = (4, 4, 6, 4)
+    ud, ipt, offs = setup_lattice(C, dims, 3)
+    u1 = O.generate(dims, 3, 81, 0.5)
+    u2 = O.generate(dims, 3, 82, 0.5)
+    v0 = O.generate(dims, 3, 83, 0.5)
+    fields = []
+    for u in (u1, u2, v0):
+        f = C.alloc_lnk()
+        write_field(f, offs, u)
+        fields.append(f)
+    C.staple_sum.argtypes = [ctypes.POINTER(ctypes.c_double)] * 2 + [ctypes.c_int] * 2 + \
+        [ctypes.POINTER(ctypes.c_double)] * 2
+    C.staple_sum(fields[0], fields[1], mu, nu, fields[2], None)
+    got = read_field(fields[2], offs)
+    want = O.staple_sum(dims, u1, u2, mu, nu, v0)
+    assert np.abs(got - want).max() < 1e-13
+    for k in range(4):
+        if k != mu:
+            assert np.array_equal(got[:, k], v0[:, k])
+    for f in fields:
+        C.free_lnk(f, 0)

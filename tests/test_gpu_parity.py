@@ -196,3 +196,24 @@ def test_measure_mixed_split_edges(wl, mwlt, mwlr):
     say("mixed split edges", mwlt, mwlr)
     got, ref = run_both(wl, (20, 6, 6, 6), 0, 7, 0.3, [0, 2, 3], mwlr, mwlt, wl.MIXED)
     assert rel_err(got, ref) < TOL_MIXED
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hyp_spatial_abelian_flux(wl, precision):
+    """The GPU 3-d HYP step (alpha2 != 0, no projection) on a constant-flux
+    abelian field equals the closed form of the reference's level wiring
+    (hyp_spatial_links.c:187-234), not the textbook one."""
+    import numpy_ref as N
+    dims = (4, 6, 6, 6)
+    u, F = N.abelian_flux_field(dims, 1, 2, 3)
+    a1, a2 = 0.75, 0.6
+    with wl.Lattice(dims) as lat:
+        lat.upload(u)
+        lat.hyp_spatial_step(a1, a2, 0, precision=precision)
+        got = lat.download()
+    want = N.scale_links(u, dims, N.hyp3_abelian_factors(F, a1, a2, "reference"))
+    other = N.scale_links(u, dims, N.hyp3_abelian_factors(F, a1, a2, "textbook"))
+    tol = 1e-14 if precision == 0 else 1e-6
+    say("hyp3 abelian", precision, max_abs(got, want))
+    assert max_abs(got, want) < tol
+    assert max_abs(got, other) > 1e-2

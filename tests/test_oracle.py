@@ -335,3 +335,20 @@ def test_wl_index_layout():
     # wloop_parms.c:431-434
     p = O.make_parms([0, 1, 2], 4, 3)
     assert O.wl_index(p, 1, 2, 3, 2, 5) == 1 + 3 * (2 + 3 * (3 + 4 * (2 + 3 * 5)))
+
+
+def test_hyp_spatial_abelian_flux_follows_reference_wiring():
+    """3-d HYP with alpha2 != 0 on a constant-flux abelian field, no projection:
+    every link is U_k times a closed-form factor that depends on which level-1
+    links level 2 reads.  Distinct fluxes per plane separate the reference's
+    wiring (links decorated in the staple's own plane, hyp_spatial_links.c:
+    187-234) from the textbook one (decorated in the orthogonal planes)."""
+    dims = (4, 6, 6, 6)
+    u, F = N.abelian_flux_field(dims, 1, 2, 3)
+    a1, a2 = 0.75, 0.6
+    got = O.hyp_spatial(dims, u, a1, a2, 0)
+    want = N.scale_links(u, dims, N.hyp3_abelian_factors(F, a1, a2, "reference"))
+    other = N.scale_links(u, dims, N.hyp3_abelian_factors(F, a1, a2, "textbook"))
+    assert np.abs(got - want).max() < 1e-14
+    assert np.abs(got - other).max() > 1e-2
+    assert np.array_equal(got[:, 0], u[:, 0])
