@@ -147,6 +147,11 @@ enum {
    WLG_TUNE_LEAN,       /* 1: least device memory (wlg_footprint_lean): fused time-slice
                            contraction for every T, time transporters kept only at the
                            starts of its T blocks, APE steps through the level buffers */
+   WLG_TUNE_MIXED_RESTART, /* mixed path, fused time-slice form for every T: 1 runs the T
+                           blocks from T = 9 in FP32, each restarted from an FP32 copy of
+                           the FP64 time transporter at its first T, so the time links past
+                           that T are accumulated in FP32 inside the open loops (0, the
+                           default: those blocks carry the open loops in FP64) */
    WLG_TUNE_COUNT
 };
 int wlg_tune(wlg_ctx *ctx, int param, int value);

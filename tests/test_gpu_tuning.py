@@ -67,6 +67,38 @@ def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel):
             assert rel_err(w, base) < (1e-13 if precision == 0 else 1e-7), (depth, starts, wg)
 
 
+# mixed path, fused form for every T with the blocks from T = 9 in FP32, each
+# restarted from an FP32 copy of the FP64 time transporter at its first T
+@pytest.mark.parametrize("dims,bc,mwlt,mwlr", [((20, 5, 6, 7), 3, 11, 3), ((18, 4, 4, 6), 0, 10, 2),
+                                               ((24, 4, 4, 4), 3, 12, 2)])
+def test_fused_restart(wl, dims, bc, mwlt, mwlr):
+    nss = [0, 2, 3]
+    u = O.generate(dims, bc, 46, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        lat.tune(wl.TUNE_MIXED_CONTRACT_KERNEL, 4)
+        lat.tune(wl.TUNE_MIXED_RESTART, 1)
+        for depth, starts, wg in ((16, 8, 8), (3, 4, 5), (8, 8, 16)):
+            lat.tune(wl.TUNE_MIXED_CONTRACT_DEPTH, depth)
+            lat.tune(wl.TUNE_MIXED_CONTRACT_STARTS, starts)
+            lat.tune(wl.TUNE_MIXED_CONTRACT_WG, wg)
+            print(f"[restart] dims={dims} bc={bc} depth={depth} starts={starts} wg={wg}", flush=True)
+            w = lat.measure(p, wl.MIXED)
+            assert rel_err(w, ref) < 1e-6, (depth, starts, wg)
+        lat.tune(wl.TUNE_LEAN, 1)
+        lat.tune(wl.TUNE_MIXED_CONTRACT_DEPTH, 0)
+        print(f"[restart] dims={dims} bc={bc} lean", flush=True)
+        assert rel_err(lat.measure(p, wl.MIXED), ref) < 1e-6
+
+
+def test_restart_rejects_bad_value(wl):
+    with wl.Lattice((4, 4, 4, 4), 3) as lat:
+        with pytest.raises(wl.WloopError, match="mixed restart"):
+            lat.tune(wl.TUNE_MIXED_RESTART, 2)
+
+
 # two-lane chain form: msl 2 and 4 (msl 3 runs the one-lane form), a spatial
 # slice that is not a multiple of the 128-site block, chains of several
 # lengths (T against N0 = 10), open and periodic boundaries.  Each trace has

@@ -4,7 +4,12 @@ For a lattice shape, boundary condition, number of operator levels, T range and
 precision, it times every candidate (contraction form x workgroup parameter,
 and for the fused time-slice form x time-slice depth x start slices per
 workgroup) on the resident field with HIP events and keeps the one with the
-least contraction time per (R,T) pair; results are cached in a JSON file keyed
+least time per (R,T) pair of the stages the form decides: the contraction
+launches and the spatial transporters (formed by a separate pass, or inside
+the first chain launch of an R: both happen once per R >= 2, in the trial as in
+the full run) per (R,T) pair of the trial, plus the time transporters (one
+chain per measurement, more of it stored for the per-(R,T) forms) spread over
+the (R,T) pairs of the full run; results are cached in a JSON file keyed
 by those properties, so a shape is tuned once per machine.  The choice changes
 only the launch shape: results agree to floating-point rounding and summation
 order (checked in tests/test_gpu_tuning.py).
@@ -33,8 +38,8 @@ DEFAULT_CACHE = os.environ.get(
 
 
 def _key(lat, parms, precision):
-    return "{}x{}x{}x{}-bc{}-msl{}-T{}-{}".format(*lat.dims, lat.bc, parms.msl, parms.mwlt,
-                                                 "mixed" if precision == W.MIXED else "fp64")
+    return "{}x{}x{}x{}-bc{}-msl{}-T{}-R{}-{}".format(*lat.dims, lat.bc, parms.msl, parms.mwlt, parms.mwlr,
+                                                      "mixed" if precision == W.MIXED else "fp64")
 
 
 def _params(precision):
@@ -71,7 +76,7 @@ def current(lat, precision):
 def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=False, log=None):
     """Pick and apply the fastest contraction launch shape; returns the choice
     {"kernel", "wg"[, "depth", "starts"], "ms_per_rt", "trials"}, ms_per_rt
-    being the contraction time per (R,T) pair."""
+    being the projected time per (R,T) pair of the stages the form decides."""
     key = _key(lat, parms, precision)
     db = _load(cache) if cache else {}
     if not force and key in db:
@@ -89,8 +94,9 @@ def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=Fals
         lat.timers_reset()
         for _ in range(2):
             lat.wloop(trial, precision)
-        _, ms = lat.contract_stats()
-        per_rt = ms / (2 * trial.mwlt * trial.mwlr)
+        st = lat.timers()
+        per_rt = ((st["contract"] + st["space_transporter"]) / (2 * trial.mwlt * trial.mwlr) +
+                  st["time_transporter"] / (2 * parms.mwlt * parms.mwlr))
         trials.append(dict(cand, ms_per_rt=per_rt))
         if log:
             log(f"[autotune] {key} {cand}: {per_rt:.3f} ms per (R,T)")

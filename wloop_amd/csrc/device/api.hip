@@ -381,21 +381,23 @@ struct Layout {
 // contraction variant: kernel form, its workgroup parameter and, for the
 // fused time-slice form, the T depth per launch and start slices per workgroup
 struct Contract {
-  int kernel, wg, depth, starts;
+  int kernel, wg, depth, starts, restart;
 };
 
 // T blocks of the fused time-slice form: [Ta, Ta+d-1] covering 1..mwlt; on the
-// mixed path the blocks of T <= 8 run in FP32, those from T = 9 in FP64
+// mixed path the blocks of T <= 8 run in FP32, those from T = 9 in FP64, or,
+// restarted (restart = 1), in FP32 from an FP32 copy of the FP64 L_Ta, so that
+// no open loop passes through more FP32 products than those of T <= 8
 struct DepthBlock {
   int Ta, d, fp64;
 };
 
-static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth) {
+static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth, int restart) {
   std::vector<DepthBlock> b;
   const int split = prec == WLG_MIXED ? (mwlt < kTimeFp64From - 1 ? mwlt : kTimeFp64From - 1) : 0;
   for (int T = 1; T <= split; T += depth) b.push_back({T, split - T + 1 < depth ? split - T + 1 : depth, 0});
   for (int T = split + 1; T <= mwlt; T += depth)
-    b.push_back({T, mwlt - T + 1 < depth ? mwlt - T + 1 : depth, 1});
+    b.push_back({T, mwlt - T + 1 < depth ? mwlt - T + 1 : depth, (prec == WLG_MIXED && restart) ? 0 : 1});
   return b;
 }
 
@@ -403,7 +405,7 @@ static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth) {
 static int depth_fp64_starts(int prec, int mwlt, int depth) {
   if (prec != WLG_MIXED) return 0;
   int n = 0;
-  for (const DepthBlock& b : depth_blocks(prec, mwlt, depth)) n += b.fp64;
+  for (const DepthBlock& b : depth_blocks(prec, mwlt, depth, 0)) n += b.fp64;
   return n;
 }
 
@@ -452,9 +454,9 @@ static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
 // remaining T running one launch per (R,T) with the chain form
 static std::vector<DepthBlock> fused_blocks(const Contract& k, int prec, int mwlt) {
   std::vector<DepthBlock> b;
-  if (k.kernel == 3) return depth_blocks(prec, mwlt, k.depth);
+  if (k.kernel == 3) return depth_blocks(prec, mwlt, k.depth, k.restart);
   if (k.kernel == 5 && prec == WLG_MIXED)
-    for (const DepthBlock& x : depth_blocks(prec, mwlt, k.depth))
+    for (const DepthBlock& x : depth_blocks(prec, mwlt, k.depth, 0))
       if (!x.fp64) b.push_back(x);
   return b;
 }
@@ -477,6 +479,7 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : 8;
+  k.restart = mixed && t[WLG_TUNE_MIXED_RESTART] == 1;
   // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
   // time links on the FP64 blocks of the mixed path)
   const int tsb = mixed ? 4 : 8;
@@ -503,9 +506,9 @@ static size_t footprint(const int dims[4], const wlg_parms* p, int prec, int lea
   // x0: the most among the library defaults and the tuning candidates
   // (streaming form: 3 x 32 chunks; chain forms: up to 2048 persistent
   // workgroups; fused form: up to 128 workgroups per start group)
-  const Contract k = fused ? Contract{3, 128, 16, 8} : Contract{1, 2048, 16, 8};
+  const Contract k = fused ? Contract{3, 128, 16, 8, 0} : Contract{1, 2048, 16, 8, 0};
   const std::vector<DepthBlock> blocks =
-      fused ? depth_blocks(prec, p->mwlt, 16) : std::vector<DepthBlock>();
+      fused ? depth_blocks(prec, p->mwlt, 16, 0) : std::vector<DepthBlock>();
   Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16), lean, blocks, fused);
   size_t hyp = 24 * slot64(L);
   size_t arena = o.total > hyp ? o.total : hyp;
@@ -758,11 +761,10 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     for (const DepthBlock& b : blocks) n32 += !b.fp64 && b.Ta > 1;
     for (int T = 2; T <= tmax; T++) {
       float2* out32 = lt32 + (size_t)(T - 1) * 9 * L.V;
-      if (o.lean_lt && T < kTimeFp64From) {  // [L_1][block starts][two in flight]
-        bool start = false;
-        for (const DepthBlock& b : blocks) start = start || (!b.fp64 && b.Ta == T);
-        out32 = lt32 + (size_t)(start ? 1 + m32++ : 1 + n32 + T % 2) * 9 * L.V;
-      }
+      bool start32 = false;  // an FP32 block starts at T
+      for (const DepthBlock& b : blocks) start32 = start32 || (!b.fp64 && b.Ta == T);
+      if (o.lean_lt)  // [L_1][block starts][two in flight]
+        out32 = lt32 + (size_t)(start32 ? 1 + m32++ : 1 + n32 + T % 2) * 9 * L.V;
       if (T < kTimeFp64From) {
         launch_time_transporter<float, float, float, float>(L, lt32, 0, lts[T - 2], out32, nullptr,
                                                             c->st);
@@ -771,7 +773,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
         bool start = false;
         for (const DepthBlock& b : blocks) start = start || (b.fp64 && b.Ta == T);
         double2* dst = start ? starts64 + (size_t)(m64++) * 9 * L.V : ch[T % 2];
-        float2* o32 = k.kernel == 3 ? nullptr : out32;
+        float2* o32 = (k.kernel == 3 && !start32) ? nullptr : out32;
         if (T == kTimeFp64From)
           launch_time_transporter<double, double, float, double>(L, U, 0, lts[T - 2], dst, o32,
                                                                  c->st);
@@ -1027,6 +1029,9 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
                    "projection)");
   if (param == WLG_TUNE_LEAN && value > 1)
     return set_err("lean layout must be 0 (default layout) or 1 (least device memory)");
+  if (param == WLG_TUNE_MIXED_RESTART && value > 1)
+    return set_err("mixed restart must be 0 (FP64 blocks from T = 9) or 1 (FP32 blocks from T = 9 "
+                   "restarted from the FP64 time transporter)");
   if ((param == WLG_TUNE_FP64_CONTRACT_WG || param == WLG_TUNE_MIXED_CONTRACT_WG) && value > 65536)
     return set_err("workgroup parameter must be 0 (default) or 1..65536");
   c->tune[param] = value;

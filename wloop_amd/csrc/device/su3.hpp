@@ -302,6 +302,18 @@ template <class R> __device__ __forceinline__ R retr_dag_mul(const M3<R>& a, con
   return r[0] + r[1] + r[2];
 }
 
+// Re tr(a^dag b) as one fused multiply-add chain over the elements in order
+template <class R> __device__ __forceinline__ R retr_dag_mul_chain(const M3<R>& a, const M3<R>& b) {
+  R t = a.re[0] * b.re[0];
+  t = fmadd(a.im[0], b.im[0], t);
+#pragma unroll
+  for (int e = 1; e < 9; e++) {
+    t = fmadd(a.re[e], b.re[e], t);
+    t = fmadd(a.im[e], b.im[e], t);
+  }
+  return t;
+}
+
 // ---- approximate SU(3) projection of hep-lat/0506008, always FP64 ---------
 // Scale to unit Frobenius norm/3, then p times
 //     X = u (3 - u^dag u)/2 ;  u = (1 - i Im det X / 3) X .

@@ -406,15 +406,18 @@ __global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, i
 // mirror), p ^ 2 and p ^ 1 (quad permutations)
 constexpr int kRowMirror = 0x140, kRowHalfMirror = 0x141, kQuadXor2 = 0x4E, kQuadXor1 = 0xB1;
 
+// (every lane of these patterns has a source lane, so the moves need no
+// fallback value)
 template <int CTRL> __device__ __forceinline__ double row_dpp(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
-__device__ __forceinline__ double lane_xor1(double v) { return row_dpp<kQuadXor1>(v); }
-__device__ __forceinline__ float lane_xor1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kQuadXor1, 0xF, 0xF, false));
+template <int CTRL> __device__ __forceinline__ float row_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
 }
+__device__ __forceinline__ double lane_xor1(double v) { return row_dpp<kQuadXor1>(v); }
+__device__ __forceinline__ float lane_xor1(float v) { return row_dpp<kQuadXor1>(v); }
 
 // ---- contraction, two-lane chain form ------------------------------------
 // As the one-lane chain form, but the msl levels of a (site, k) pair are split
@@ -718,34 +721,54 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     const long kV = (long)9 * k * V;
 
     // one time slice of the block (sinks S_j and the links of usrc at x and
-    // x + R k) into registers / from registers into a ring slot
+    // x + R k) into registers / from registers into a ring slot.  Offsets and
+    // validity of this thread's elements are fixed per pair block.  Every
+    // thread loads every one of its element positions unconditionally (a
+    // position with no element, or past the end of the slice, reads a valid
+    // link: element 0 of its plane), and the store into the ring zeroes what
+    // is past the end of the slice and skips positions with no element, so the
+    // loads sit in no branch and are waited for only at the store, one step
+    // later.
     v2<TS> ps[PS];
     v2<TC> pu[PU];
-    auto fetch_at = [&](long ys, const v2<TC>* __restrict__ usrc) {
+    long so_[PS], uoff_[PU];
+    bool sok_[PS], uok_[PU];
+    auto offsets = [&]() {
 #pragma unroll
       for (int m = 0; m < PS; m++) {
-        ps[m].x = TS(0);
-        ps[m].y = TS(0);
-        if (sj_[m] >= 0 && xs0 + sp_[m] < slice) ps[m] = sb_[m][kV + ys + xs0];
+        sok_[m] = sj_[m] >= 0 && xs0 + sp_[m] < slice;
+        so_[m] = kV + (sok_[m] ? xs0 : -(long)sp_[m]);
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
-        pu[m].x = TC(0);
-        pu[m].y = TC(0);
-        if (uh_[m] >= 0 && xs0 + up_[m] < slice)
-          pu[m] = usrc[uo_[m] + ys + (uh_[m] ? xr_tab[up_[m]] : xs0 + up_[m])];
+        uok_[m] = uh_[m] >= 0 && xs0 + up_[m] < slice;
+        uoff_[m] = uo_[m] + (uok_[m] ? (uh_[m] > 0 ? xr_tab[up_[m]] : xs0 + up_[m]) : 0);
       }
+    };
+    auto fetch_at = [&](long ys, const v2<TC>* __restrict__ usrc) {
+#pragma unroll
+      for (int m = 0; m < PS; m++) ps[m] = sb_[m][so_[m] + ys];
+#pragma unroll
+      for (int m = 0; m < PU; m++) pu[m] = usrc[uoff_[m] + ys];
     };
     // ring offset o: time slice x0g+Ta+o
     auto fetch = [&](int o) { fetch_at((long)((x0g + Ta + o) % N0) * slice, U0); };
     auto put = [&](int slot) {
       char* dst = ring + slot * SLOT;
 #pragma unroll
-      for (int m = 0; m < PS; m++)
-        if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT] = ps[m];
+      for (int m = 0; m < PS; m++) {
+        v2<TS> v;
+        v.x = sok_[m] ? ps[m].x : TS(0);
+        v.y = sok_[m] ? ps[m].y : TS(0);
+        if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT] = v;
+      }
 #pragma unroll
-      for (int m = 0; m < PU; m++)
-        if (uh_[m] >= 0) ((v2<TC>*)(dst + NS * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = pu[m];
+      for (int m = 0; m < PU; m++) {
+        v2<TC> v;
+        v.x = uok_[m] ? pu[m].x : TC(0);
+        v.y = uok_[m] ? pu[m].y : TC(0);
+        if (uh_[m] >= 0) ((v2<TC>*)(dst + NS * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = v;
+      }
     };
     // (up to three slices in flight, as the register budget of the occupancy
     // target allows)
@@ -753,6 +776,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
 
     if (threadIdx.x < 16) xr_tab[threadIdx.x] = xr;
     __syncthreads();  // xr_tab; the previous block's last reads of the ring
+    offsets();
     // the start slices x0g+w with S_i(x) and L_Ta at x and x + R k: for Ta = 1
     // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
     // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
@@ -820,7 +844,8 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
       const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
-      // traces against the sinks S_j(x + (Ta+n) 0)
+      // traces against the sinks S_j(x + (Ta+n) 0): products in the compute
+      // type, each trace converted to FP64 before it is summed
       double tr[4];
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -833,10 +858,11 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
             sj.re[e] = v.x;
             sj.im[e] = v.y;
           }
-          tr[j] = (double)retr_dag_mul(convert<TC>(sj), D);
+          tr[j] = (double)retr_dag_mul_chain(convert<TC>(sj), D);
         }
       }
-      // sum over the 16 pairs of the row; lane p ends with the sum for j = p >> 2
+      // sum over the 16 pairs of the row in FP64; lane p ends with the sum
+      // for j = p >> 2
       const bool b3 = (p & 8) != 0, b2 = (p & 4) != 0;
       double s0 = b3 ? tr[0] : tr[2], s1 = b3 ? tr[1] : tr[3];
       double k0 = b3 ? tr[2] : tr[0], k1 = b3 ? tr[3] : tr[1];
