@@ -219,9 +219,11 @@ def cpu_baseline(wl_cfg, parms_o):
 
 def cpu_baseline_full(log_out):
     """Whole ms10 measurements of the 8^4 and 24^3x48 workloads by the oracle
-    (reference loop structure), per build and thread count (the 24^3x48 one at
-    one thread is left out: about 30 minutes)."""
+    (reference loop structure), per build and thread count; the oracle reports
+    the end of its stages on stderr meanwhile (a one-thread 24^3x48 run takes
+    minutes)."""
     import oracle as O
+    os.environ["WO_PROGRESS"] = "1"
     model, ncpu = host_cpu()
     res = {"host_cpu": model, "threads_all": ncpu, "runs": []}
     for name in ("c1", "c2"):
@@ -230,8 +232,7 @@ def cpu_baseline_full(log_out):
         u = O.generate(cfg["dims"], cfg["bc"], cfg["seed"], cfg["eps"])
         for variant in ("refflags", "v3"):
             for threads in (1, ncpu):
-                if name == "c2" and threads == 1:
-                    continue
+                log_out(f"[bench] cpu full {name} {variant} {threads} threads: start")
                 t, _ = O.ms10_timed(cfg["dims"], cfg["bc"], u, ALPHA_ACTION, p, threads, variant=variant)
                 r = {"workload": cfg["name"], "build": variant, "threads": threads, "seconds": t,
                      "configs_per_s": 1.0 / t}

@@ -37,6 +37,7 @@
 #include "wl_oracle.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
@@ -410,6 +411,15 @@ int wo_wl_idx(const wo_parms *p, int sl1, int sl2, int t, int r, int x0)
    return sl1 + p->msl * (sl2 + p->msl * (t + p->mwlt * (r + p->mwlr * x0)));
 }
 
+/* Progress of long runs on stderr (stage ends), when WO_PROGRESS is set. */
+static void progress(const char *what, int n)
+{
+   if (getenv("WO_PROGRESS")) {
+      fprintf(stderr, "[oracle] %s %d done\n", what, n);
+      fflush(stderr);
+   }
+}
+
 static void build_levels(const geom *g, const double *u, const wo_parms *p, double *lev)
 {
    size_t fsz = sizeof(double) * 18 * 4 * g->vol;
@@ -428,6 +438,7 @@ static void build_levels(const geom *g, const double *u, const wo_parms *p, doub
       for (long x = 0; x < g->vol; x++)
          for (int k = 0; k < 3; k++)
             memcpy(l + 18 * (3 * x + k), L4(f, x, k + 1), sizeof(su3));
+      progress("operator level", sl);
    }
    free(work);
    free(f);
@@ -575,6 +586,7 @@ void wo_wloop(const int *dims, const double *u, const wo_parms *p, int mode, int
          }
          contract_rt(&g, p, r, t, lt, (R > 1) ? slong : lev, mode, acc, wl);
       }
+      progress("R", R);
    }
    free(acc); free(ltmp); free(lt); free(stmp); free(slong); free(lev);
    geom_free(&g);
@@ -658,6 +670,7 @@ void wo_ms10(const int *dims, int bc, double *u, const double *aa, const wo_parm
    if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
    wo_hyp(dims, bc, u, aa[0], aa[1], aa[2], p->proj, p->proj_iter, 1, 1);  /* ms10.c:640 */
+   progress("hyp", 1);
    wo_wloop(dims, u, p, mode, nthreads, wl);                                 /* ms10.c:642 */
 #ifdef _OPENMP
    omp_set_num_threads(saved);
