@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void contract_chain(Lat L, int R, int T, int g
 // every S_R is formed once (the start slice of a chain twice, by the same
 // lane).  Requires msl <= 4.
 template <int MSL, class TS, class TC, bool BUILD>
-__global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, int g,
+__global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, int g, int gb,
                                                        const v2<TS>* __restrict__ LT,
                                                        LevelPtrs<TS> S, BuildPtrs<TS> B,
                                                        double* __restrict__ partial) {
@@ -305,6 +305,8 @@ __global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, i
   int par = 0;
   for (long u = xcd_order(blockIdx.x, gridDim.x); u < units; u += gridDim.x) {
     const int c = (int)(u / nblocks);
+    // chain c: segment c / gb of the base chain c % gb (slices = c % gb mod gb)
+    const int cs = (int)(((long)(c % gb) + (long)(c / gb) * len * T) % N0);
     const long bl = u % nblocks;
     const int k = (int)(bl % 3);
     const long xs0 = (bl / 3) * 256 + threadIdx.x;
@@ -326,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, i
     };
     M3<TC> D[MSL], Sx[MSL];
     {
-      const long s0 = (long)c * slice + xs;
+      const long s0 = (long)cs * slice + xs;
       M3<TS> a, b, t;
       load(a, LT, L.V, 0, s0);
       load(b, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
@@ -337,7 +339,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, i
         D[i] = mul(dag_mul(ac, convert<TC>(t)), bc);
       }
     }
-    int x0 = c;
+    int x0 = cs;
     int x1 = x0 + T;
     if (x1 >= N0) x1 -= N0;
 #pragma unroll
@@ -436,7 +438,7 @@ __device__ __forceinline__ float lane_xor1(float v) { return row_dpp<kQuadXor1>(
 // at every step (4.0 against 4.2 ms per (R,T) in FP64 at 48^3x96).  Requires
 // msl = 2 or 4.
 template <int MSL, class TS, class TC, bool WACC, bool BUILD>
-__global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, int g,
+__global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, int g, int gb,
                                                           const v2<TS>* __restrict__ LT,
                                                           LevelPtrs<TS> S, BuildPtrs<TS> B,
                                                           double* __restrict__ partial) {
@@ -488,6 +490,8 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
   int par = 0;
   for (long u = xcd_order(blockIdx.x, gridDim.x); u < units; u += gridDim.x) {
     const int c = (int)(u / nblocks);
+    // chain c: segment c / gb of the base chain c % gb (slices = c % gb mod gb)
+    const int cs = (int)(((long)(c % gb) + (long)(c / gb) * len * T) % N0);
     const long bl = u % nblocks;
     const int k = (int)(bl % 3);
     const long xs0 = (bl / 3) * 128 + (threadIdx.x >> 1);
@@ -510,7 +514,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
     };
     M3<TC> D[MH], Sx[MH];
     {
-      const long s0 = (long)c * slice + xs;
+      const long s0 = (long)cs * slice + xs;
       M3<TS> a, b, t;
       load(a, LT, L.V, 0, s0);
       load(b, LT, L.V, 0, fwd(L, s0, x, k + 1, R));
@@ -521,7 +525,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
         D[m] = mul(dag_mul(ac, convert<TC>(t)), bc);
       }
     }
-    int x0 = c;
+    int x0 = cs;
     int x1 = x0 + T;
     if (x1 >= N0) x1 -= N0;
 #pragma unroll
@@ -1006,42 +1010,69 @@ static void stream_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
                      R, T, g, LT, sp, partial);
 }
 
+// Chains of a per-(R,T) chain launch: the gb = gcd(T, N0) chains x0 -> x0+T
+// -> ... of N0/gb slices each, split into the number of equal segments (each
+// restarted from its own first slice, which costs one more slice of loads)
+// that keeps the persistent workgroups busiest: work units (blocks of one
+// segment) are taken in rounds of nwg, so too few units leave workgroups idle
+// and a last partial round leaves most of them idle at the end.
+static int chain_count(int N0, int gb, long blocks, int nwg) {
+  const int lb = N0 / gb;
+  int best = 1;
+  double beff = 0.0;
+  for (int s = 1; s <= lb; s++) {
+    if (lb % s) continue;
+    const int len = lb / s;
+    if (len < 2 && s > 1) break;
+    const long u = blocks * gb * s;
+    const long rounds = (u + nwg - 1) / nwg;
+    const double eff = (double)u / ((double)rounds * nwg) * (double)len / (len + 1);
+    if (eff > beff * 1.01) {  // a split must gain at least 1%
+      beff = eff;
+      best = s;
+    }
+  }
+  return gb * best;
+}
+
 template <int MSL, class TS, class TC>
-static void chain1_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+static void chain1_launch(const Lat& L, int R, int T, int gb, const v2<TS>* LT,
                           const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st,
                           const BuildPtrs<TS>* build = nullptr) {
   size_t lds = (size_t)L.n[0] * MSL * MSL * sizeof(double) + 2 * 4 * 16 * sizeof(double) +
                4 * 64 * 16 * sizeof(TC);
+  const int g = chain_count(L.n[0], gb, 3 * ((L.st[0] + 255) / 256), nwg);
   if (build)
     hipLaunchKernelGGL((contract_chain1<MSL, TS, TC, true>), dim3(nwg), dim3(256), lds, st, L, R, T, g,
-                       LT, sp, *build, partial);
+                       gb, LT, sp, *build, partial);
   else
     hipLaunchKernelGGL((contract_chain1<MSL, TS, TC, false>), dim3(nwg), dim3(256), lds, st, L, R, T, g,
-                       LT, sp, BuildPtrs<TS>{}, partial);
+                       gb, LT, sp, BuildPtrs<TS>{}, partial);
 }
 
 template <int MSL, class TS, class TC>
-static void chain2_launch(const Lat& L, int R, int T, int g, const v2<TS>* LT,
+static void chain2_launch(const Lat& L, int R, int T, int gb, const v2<TS>* LT,
                           const LevelPtrs<TS>& sp, double* partial, int nwg, hipStream_t st,
                           const BuildPtrs<TS>* build = nullptr) {
   // per-wave accumulators where 4 of them fit into LDS (N0 <= 300 at msl 4)
   const size_t rest = 2 * 4 * 16 * sizeof(double) + 4 * 32 * 16 * sizeof(TC);
   const size_t acc1 = (size_t)L.n[0] * MSL * MSL * sizeof(double);
   const BuildPtrs<TS> b = build ? *build : BuildPtrs<TS>{};
+  const int g = chain_count(L.n[0], gb, 3 * ((L.st[0] + 127) / 128), nwg);
   if (4 * acc1 + rest <= 163840) {
     if (build)
       hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true, true>), dim3(nwg), dim3(256), 4 * acc1 + rest,
-                         st, L, R, T, g, LT, sp, b, partial);
+                         st, L, R, T, g, gb, LT, sp, b, partial);
     else
       hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, true, false>), dim3(nwg), dim3(256), 4 * acc1 + rest,
-                         st, L, R, T, g, LT, sp, b, partial);
+                         st, L, R, T, g, gb, LT, sp, b, partial);
   } else {
     if (build)
       hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false, true>), dim3(nwg), dim3(256), acc1 + rest, st,
-                         L, R, T, g, LT, sp, b, partial);
+                         L, R, T, g, gb, LT, sp, b, partial);
     else
       hipLaunchKernelGGL((contract_chain2<MSL, TS, TC, false, false>), dim3(nwg), dim3(256), acc1 + rest, st,
-                         L, R, T, g, LT, sp, b, partial);
+                         L, R, T, g, gb, LT, sp, b, partial);
   }
 }
 
