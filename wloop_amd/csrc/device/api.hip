@@ -166,12 +166,7 @@ extern "C" int wlg_create(wlg_ctx** out, const int dims[4], int bc, int device) 
   wlg_ctx* c = new wlg_ctx();
   c->device = device;
   c->bc = bc;
-  for (int m = 0; m < 4; m++) c->L.n[m] = dims[m];
-  c->L.st[3] = 1;
-  c->L.st[2] = dims[3];
-  c->L.st[1] = (long)dims[2] * dims[3];
-  c->L.st[0] = (long)dims[1] * dims[2] * dims[3];
-  c->L.V = (long)dims[0] * c->L.st[0];
+  lat_set(c->L, dims);
   if (c->L.V >= (1L << 28)) {  // 32-bit byte offsets of a site within a field plane
     const long V = c->L.V;
     delete c;
@@ -495,12 +490,7 @@ static size_t partial_count(const Lat& L, const wlg_parms* p, const Contract& k)
 
 static size_t footprint(const int dims[4], const wlg_parms* p, int prec, int lean) {
   Lat L;
-  for (int m = 0; m < 4; m++) L.n[m] = dims[m];
-  L.st[3] = 1;
-  L.st[2] = dims[3];
-  L.st[1] = (long)dims[2] * dims[3];
-  L.st[0] = (long)dims[1] * dims[2] * dims[3];
-  L.V = (long)dims[0] * L.st[0];
+  lat_set(L, dims);
   const bool fused = lean && p->msl <= 4;
   // default launch shapes: depth-16 blocks of the fused form; partial sums per
   // x0: the most among the library defaults and the tuning candidates

@@ -24,7 +24,7 @@ namespace wl {
 __device__ __forceinline__ site_t stencil_site(int ny, int& y) {
   const int b = xcd_order(blockIdx.x, gridDim.x);
   y = b % ny;
-  return (site_t)(b / ny) * blockDim.x + threadIdx.x;
+  return (site_t)(b / ny) * 256 + threadIdx.x;  // every stencil launch has 256 threads
 }
 
 // v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
@@ -278,7 +278,7 @@ __device__ __forceinline__ void site_staples(M3<TC> S[3], const Lat& L, site_t s
 }
 
 __device__ __forceinline__ site_t block_site() {
-  return (site_t)xcd_order(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  return (site_t)xcd_order(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
 }
 
 template <class TC, class TS>

@@ -458,13 +458,40 @@ struct Lat {
   int n[4];
   long st[4];  // strides: N1N2N3, N2N3, N3, 1
   long V;
+  // division by n[m] of a site index r < 2^31: floor(r / n) =
+  // mulhi(r, mg) >> sh, or r >> sh where n is a power of two (mg = 0)
+  uint32_t mg[4], sh[4];
 };
 
+// extents, strides and division constants of a lattice.  For n not a power of
+// two, sh = floor(log2 n) and mg = ceil(2^(32+sh) / n) < 2^32; with
+// mg n = 2^(32+sh) + e, 0 <= e < n < 2^(sh+1), r mg / 2^(32+sh) differs from
+// r / n by r e / (n 2^(32+sh)) < 1/n for r < 2^31, so the floor is exact.
+inline void lat_set(Lat& L, const int dims[4]) {
+  for (int m = 0; m < 4; m++) {
+    L.n[m] = dims[m];
+    const uint32_t d = (uint32_t)dims[m];
+    uint32_t l = 0;
+    while (l < 31 && (2u << l) <= d) l++;
+    L.sh[m] = l;
+    L.mg[m] = (d & (d - 1)) == 0 ? 0u : (uint32_t)((((uint64_t)1 << (32 + l)) + d - 1) / d);
+  }
+  L.st[3] = 1;
+  L.st[2] = dims[3];
+  L.st[1] = (long)dims[2] * dims[3];
+  L.st[0] = (long)dims[1] * dims[2] * dims[3];
+  L.V = (long)dims[0] * L.st[0];
+}
+
+__device__ __forceinline__ uint32_t div_n(const Lat& L, int m, uint32_t r) {
+  return L.mg[m] ? (__umulhi(r, L.mg[m]) >> L.sh[m]) : (r >> L.sh[m]);
+}
+
 __device__ __forceinline__ void coords(const Lat& L, site_t s, int x[4]) {
-  site_t r = s;
-  x[3] = (int)(r % (site_t)L.n[3]); r /= (site_t)L.n[3];
-  x[2] = (int)(r % (site_t)L.n[2]); r /= (site_t)L.n[2];
-  x[1] = (int)(r % (site_t)L.n[1]); r /= (site_t)L.n[1];
+  uint32_t r = s, q;
+  q = div_n(L, 3, r); x[3] = (int)(r - q * (uint32_t)L.n[3]); r = q;
+  q = div_n(L, 2, r); x[2] = (int)(r - q * (uint32_t)L.n[2]); r = q;
+  q = div_n(L, 1, r); x[1] = (int)(r - q * (uint32_t)L.n[1]); r = q;
   x[0] = (int)r;
 }
 
