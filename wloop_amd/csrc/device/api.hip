@@ -256,14 +256,14 @@ static int run_hyp(wlg_ctx* c, const double2* in, double2* out, const double a[3
   }
   if (a[0] == 0.0) return 0;  // hyp_links.c:405-406
   if (scratch(c, 24 * s)) return -1;
-  double2* d1 = (double2*)c->arena;
-  double2* d2 = (double2*)(c->arena + 12 * s);
+  // decorated links of levels 1 and 2: 12 slots each (FP32 on the mixed path)
   if (prec == WLG_MIXED)
-    launch_hyp<float, double>(c->L, in, out, d1, d2, a, proj, piter, time, space, c->bc == 0,
-                              c->st);
+    launch_hyp<float, double, float>(c->L, in, out, (float2*)c->arena, (float2*)(c->arena + 12 * s),
+                                     a, proj, piter, time, space, c->bc == 0, c->st);
   else
-    launch_hyp<double, double>(c->L, in, out, d1, d2, a, proj, piter, time, space, c->bc == 0,
-                               c->st);
+    launch_hyp<double, double, double>(c->L, in, out, (double2*)c->arena,
+                                       (double2*)(c->arena + 12 * s), a, proj, piter, time, space,
+                                       c->bc == 0, c->st);
   CKL();
   if (c->bc == 0 && !time) {
     launch_zero_bc(c->L, out, c->st);  // set_bc (hyp_links.c:447)

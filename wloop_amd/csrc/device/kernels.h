@@ -7,8 +7,9 @@
 namespace wl {
 
 // smear.hip
-template <class TC, class TS>
-void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* D2,
+// TD: storage of the decorated links of levels 1 and 2 (D1, D2)
+template <class TC, class TS, class TD>
+void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TD>* D1, v2<TD>* D2,
                 const double a[3], int proj, int piter, int time, int space, int open_bc,
                 hipStream_t st);
 template <class TC, class TS>

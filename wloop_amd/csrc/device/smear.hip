@@ -52,10 +52,11 @@ __device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, site_t s, co
   dag_mul_acc(v, convert<TC>(a), t);
 }
 
-// out = P((1-a)U + b S), S in TC, U and out in TS, combination in FP64
-template <class TC, class TS>
+// out = P((1-a)U + b S), S in TC, U in TS, out in TO (TS unless given),
+// combination in FP64
+template <class TC, class TS, class TO = TS>
 __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __restrict__ U, int lu,
-                                               v2<TS>* __restrict__ out, int lo, long V, site_t s,
+                                               v2<TO>* __restrict__ out, int lo, long V, site_t s,
                                                double alpha, double beta, int proj, int piter) {
   M3<TS> u;
   load_u(u, U, V, lu, s);
@@ -66,7 +67,7 @@ __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __
     w.im[e] = alpha * (double)u.im[e] + beta * (double)S.im[e];
   }
   if (proj) project(w, piter);
-  store_u(convert<TS>(w), out, V, lo, s);
+  store_u(convert<TO>(w), out, V, lo, s);
 }
 
 __device__ __forceinline__ int part4(int a, int b) {
@@ -76,11 +77,15 @@ __device__ __forceinline__ int part4(int a, int b) {
   return d - 1;
 }
 
+// The decorated links of levels 1 and 2 are stored in TD: the link type TS of
+// the field, or FP32 on the mixed path (the staple sums that read them run in
+// FP32 there).
+//
 // Level 1: V1[part{mu,eta}]_mu = P((1-a3) U_mu + a3/2 S_{mu eta}[U,U]);
 // kind y = 3*mu + (i-1), eta = (mu+i)&3.  Output slot part*4+mu.
-template <class TC, class TS>
+template <class TC, class TS, class TD>
 __global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restrict__ U,
-                                                  v2<TS>* __restrict__ D1, double alpha,
+                                                  v2<TD>* __restrict__ D1, double alpha,
                                                   double beta, int proj, int piter) {
   int y;
   const site_t s = stencil_site(12, y);
@@ -91,15 +96,15 @@ __global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restric
   M3<TC> S;
   set_zero(S);
   staple_add<TC, TS>(S, L, s, x, U, eta, U, mu, mu, eta);
-  decorate_store<TC, TS>(S, U, mu, D1, 4 * part4(mu, eta) + mu, L.V, s, alpha, beta, proj, piter);
+  decorate_store<TC, TS, TD>(S, U, mu, D1, 4 * part4(mu, eta) + mu, L.V, s, alpha, beta, proj, piter);
 }
 
 // Level 2: V2[part{mu,nu}]_mu = P((1-a2)U_mu + a2/4 sum_rho S_{mu rho}[A,B]),
 // rho-links A = V1[part{mu,nu}], mu-links B = V1[part{nu,rho}], rho ascending.
-template <class TC, class TS>
+template <class TC, class TS, class TD>
 __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restrict__ U,
-                                                  const v2<TS>* __restrict__ D1,
-                                                  v2<TS>* __restrict__ D2, double alpha,
+                                                  const v2<TD>* __restrict__ D1,
+                                                  v2<TD>* __restrict__ D2, double alpha,
                                                   double beta, int proj, int piter) {
   int y;
   const site_t s = stencil_site(12, y);
@@ -112,17 +117,17 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
   set_zero(S);
   for (int rho = 0; rho < 4; rho++) {
     if (rho == mu || rho == nu) continue;
-    staple_add<TC, TS>(S, L, s, x, D1, 4 * n + rho, D1, 4 * part4(nu, rho) + mu, mu, rho);
+    staple_add<TC, TD>(S, L, s, x, D1, 4 * n + rho, D1, 4 * part4(nu, rho) + mu, mu, rho);
   }
-  decorate_store<TC, TS>(S, U, mu, D2, 4 * n + mu, L.V, s, alpha, beta, proj, piter);
+  decorate_store<TC, TS, TD>(S, U, mu, D2, 4 * n + mu, L.V, s, alpha, beta, proj, piter);
 }
 
 // Level 3: V_mu = P((1-a1)U_mu + a1/6 sum_{i=1..3} S_{mu nu}[V2[part{nu,mu}]]),
 // nu = (mu+i)&3, written to out (which may be U itself: each thread reads
 // only its own U_mu(x) besides D2).  Kind y = mu - mu0 of ny.
-template <class TC, class TS>
+template <class TC, class TS, class TD>
 __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>* out,
-                                                  const v2<TS>* __restrict__ D2, int mu0, int ny,
+                                                  const v2<TD>* __restrict__ D2, int mu0, int ny,
                                                   double alpha, double beta, int proj,
                                                   int piter, int open_bc) {
   int y;
@@ -136,7 +141,7 @@ __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>
   for (int i = 1; i < 4; i++) {
     int nu = (mu + i) & 3;
     int n = part4(nu, mu);
-    staple_add<TC, TS>(S, L, s, x, D2, 4 * n + nu, D2, 4 * n + mu, mu, nu);
+    staple_add<TC, TD>(S, L, s, x, D2, 4 * n + nu, D2, 4 * n + mu, mu, nu);
   }
   decorate_store<TC, TS>(S, U, mu, out, mu, L.V, s, alpha, beta, proj, piter);
   if (open_bc && mu == 0 && x[0] == L.n[0] - 1) {
@@ -390,18 +395,18 @@ __global__ __launch_bounds__(256) void staple_sum_kernel(Lat L, const double2* _
 // (stencil_site)
 static inline dim3 grid_for(long V, int ny) { return dim3((unsigned)(ny * ((V + 255) / 256)), 1, 1); }
 
-template <class TC, class TS>
-void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TS>* D1, v2<TS>* D2,
+template <class TC, class TS, class TD>
+void launch_hyp(const Lat& L, const v2<TS>* U, v2<TS>* out, v2<TD>* D1, v2<TD>* D2,
                 const double a[3], int proj, int piter, int time, int space, int open_bc,
                 hipStream_t st) {
-  hipLaunchKernelGGL((hyp_level1<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1,
+  hipLaunchKernelGGL((hyp_level1<TC, TS, TD>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1,
                      1.0 - a[2], a[2] / 2.0, proj, piter);
-  hipLaunchKernelGGL((hyp_level2<TC, TS>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1, D2,
+  hipLaunchKernelGGL((hyp_level2<TC, TS, TD>), grid_for(L.V, 12), dim3(256), 0, st, L, U, D1, D2,
                      1.0 - a[1], a[1] / 4.0, proj, piter);
   int mu0 = 1 - (time > 0), mu1 = 1 + 3 * (space > 0);
   if (mu1 > mu0)
-    hipLaunchKernelGGL((hyp_level3<TC, TS>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U, out,
-                       D2, mu0, mu1 - mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
+    hipLaunchKernelGGL((hyp_level3<TC, TS, TD>), grid_for(L.V, mu1 - mu0), dim3(256), 0, st, L, U,
+                       out, D2, mu0, mu1 - mu0, 1.0 - a[0], a[0] / 6.0, proj, piter, open_bc);
 }
 
 template <class TC, class TS>
@@ -449,10 +454,12 @@ void launch_staple_sum(const Lat& L, const double2* A, const double2* B, double2
   hipLaunchKernelGGL(staple_sum_kernel, grid_for(L.V, 1), dim3(256), 0, st, L, A, B, v, mu, nu);
 }
 
-template void launch_hyp<double, double>(const Lat&, const double2*, double2*, double2*, double2*,
-                                         const double*, int, int, int, int, int, hipStream_t);
-template void launch_hyp<float, double>(const Lat&, const double2*, double2*, double2*, double2*,
-                                        const double*, int, int, int, int, int, hipStream_t);
+template void launch_hyp<double, double, double>(const Lat&, const double2*, double2*, double2*,
+                                                  double2*, const double[3], int, int, int, int,
+                                                  int, hipStream_t);
+template void launch_hyp<float, double, float>(const Lat&, const double2*, double2*, float2*,
+                                               float2*, const double[3], int, int, int, int, int,
+                                               hipStream_t);
 template void launch_hyp3<double, double>(const Lat&, const double2*, double2*, double2*, double,
                                           double, int, int, hipStream_t);
 template void launch_hyp3<float, double>(const Lat&, const double2*, double2*, double2*, double,
