@@ -95,10 +95,10 @@ def ape_flop_per_site(p):
 KERNEL_LABELS = {
     ("ape_step", "mixed"): "ape_step<float, float>",
     ("ape_step", "fp64"): "ape_step<double, double>",
-    ("hyp", "mixed"): "hyp_level1/2/3<float, double> (3 launches)",
-    ("hyp", "fp64"): "hyp_level1/2/3<double, double> (3 launches)",
-    ("fused", "mixed"): "contract_depth<4, 8, float, float> (fused time slices, FP32 blocks)",
-    ("fused", "fp64"): "contract_depth<4, 8, double, double> (fused time slices)",
+    ("hyp", "mixed"): "hyp_level1/2/3<float, double, float> (3 launches)",
+    ("hyp", "fp64"): "hyp_level1/2/3<double, double, double> (3 launches)",
+    ("fused", "mixed"): "contract_depth<4, {W}, float, float> (fused time slices, FP32 blocks)",
+    ("fused", "fp64"): "contract_depth<4, {W}, double, double> (fused time slices)",
     ("chain", "mixed"): "contract_chain1<4, float, float, false> (one-lane chain)",
     ("chain_build", "mixed"): "contract_chain1<4, float, float, true> (one-lane chain forming S_R)",
     ("chain", "fp64"): "contract_chain2<4, double, double, *, false> (two-lane chain)",
@@ -106,7 +106,7 @@ KERNEL_LABELS = {
 }
 
 
-def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile):
+def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8):
     """One roofline entry per launch kind of a path: the APE step and HYP
     against the vector FLOP peak (FP32 without packed math and FP64 issue at
     the same rate), the fused time-slice launches against the vector FLOP peak,
@@ -115,7 +115,8 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile):
     events on the context's stream around each group of launches; the APE step
     from the levels stage over its step count, HYP from its stage per
     configuration); traffic = the HBM bytes per launch of the committed rocprof
-    summary, where it has the kind."""
+    summary, where it has the kind.  starts: start slices per workgroup of
+    the fused launches (their template parameter W)."""
     link = 72 if prec == "mixed" else 144
     msl = parms.msl
     nsteps = sum(parms.nss[:msl])
@@ -135,7 +136,7 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile):
             e = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": ach / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": per_launch}
         tb = traffic.get(key or f"{kind}/{prec}")
-        e.update({"traffic": tb, "kernel": KERNEL_LABELS.get((kind, prec), kind), "kind": kind,
+        e.update({"traffic": tb, "kernel": KERNEL_LABELS.get((kind, prec), kind).format(W=starts), "kind": kind,
                   "avg_launch_ms": ms_per_launch, "launches_per_config": launches_per_cfg,
                   "ms_per_config": ms_per_launch * launches_per_cfg})
         if tb:
@@ -494,10 +495,14 @@ def main():
     if os.path.exists(prof_path) and args.workload == "c3":
         with open(prof_path) as f:
             profile = json.load(f)
-    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile),
+    def starts(name):  # start slices of the fused launches of a path (0: library default 8)
+        return (choice.get(name) or {}).get("starts") or 8
+
+    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile, starts("mixed")),
                  "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, profile)}
     if best64:
-        rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps, profile)
+        rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps, profile,
+                                                  starts("fp64"))
         best64["roofline"] = max(rooflines["fp64_best"], key=lambda e: e["ms_per_config"])
     primary = max(rooflines["mixed"], key=lambda e: e["ms_per_config"])
 
