@@ -97,8 +97,9 @@ KERNEL_LABELS = {
     ("ape_step", "fp64"): "ape_step<double, double>",
     ("hyp", "mixed"): "hyp_level1/2/3<float, double, float> (3 launches)",
     ("hyp", "fp64"): "hyp_level1/2/3<double, double, double> (3 launches)",
-    ("fused", "mixed"): "contract_depth<4, {W}, float, float> (fused time slices, FP32 blocks)",
-    ("fused", "fp64"): "contract_depth<4, {W}, double, double> (fused time slices)",
+    ("fused", "mixed"): "contract_depth<4, {W}, float, float, false> (fused time slices, FP32 blocks)",
+    ("fused", "fp64"): "contract_depth<4, {W}, double, double, false> (fused time slices)",
+    ("stored", "mixed"): "contract_depth<4, {W}, float, float, true> (fused over the stored transporters)",
     ("chain", "mixed"): "contract_chain1<4, float, float, false> (one-lane chain)",
     ("chain_build", "mixed"): "contract_chain1<4, float, float, true> (one-lane chain forming S_R)",
     ("chain", "fp64"): "contract_chain2<4, double, double, *, false> (two-lane chain)",
@@ -150,10 +151,15 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8):
     n_chain = lstats["chain"][0] / steps
     n_build = lstats["chain_build"][0] / steps
     n_fused = lstats["fused"][0] / steps
+    n_stored = lstats.get("stored", (0, 0.0))[0] / steps
+    # T values per launch: the stored blocks cover T = 10 .. mwlt of every R
+    t_stored = max(parms.mwlt - 9, 0) * parms.mwlr / n_stored if n_stored else 0
+    flop_per_t = V * 3 * (2 * msl * 198 + msl * msl * 36)
     if n_fused:
-        t_per_launch = (pairs - n_chain - n_build) / n_fused
-        flop = V * 3 * (2 * msl * 198 + msl * msl * 36) * t_per_launch
-        entry("fused", flop, "flop", n_fused, lstats["fused"][1] / lstats["fused"][0])
+        t_per_launch = (pairs - n_chain - n_build - n_stored * t_stored) / n_fused
+        entry("fused", flop_per_t * t_per_launch, "flop", n_fused, lstats["fused"][1] / lstats["fused"][0])
+    if n_stored:
+        entry("stored", flop_per_t * t_stored, "flop", n_stored, lstats["stored"][1] / lstats["stored"][0])
     if n_chain:
         entry("chain", V * (1 + 3 * msl) * link, "bytes", n_chain, lstats["chain"][1] / lstats["chain"][0])
     if n_build:

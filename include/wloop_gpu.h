@@ -128,7 +128,10 @@ size_t wlg_footprint_lean(const int dims[4], const wlg_parms *p, int precision);
  * 4 fused time-slice form, msl <= 4, 5 chain form with the levels of a
  * (site,k) split over two lanes, msl = 2 or 4, else the one-lane form, 6 the
  * fused form for the FP32 T blocks (T <= 8) of the mixed path and the one-lane
- * chain form for the T beyond, msl <= 4; on the FP64 path form 6 is form 5) and
+ * chain form for the T beyond, msl <= 4, 7 as 6 with the one-lane chain form
+ * at T = 9 only and the fused form for T >= 10 forming the open loops afresh at
+ * every T from the stored FP32 copies of the FP64 time transporters; on the
+ * FP64 path forms 6 and 7 are form 5) and
  * its workgroup parameter (chunks per
  * (x0,k) for the streaming form, persistent workgroups for the chain forms,
  * workgroups per group of start slices for the fused form; form 6 uses it for
@@ -167,9 +170,10 @@ double wlg_timer_ms(wlg_ctx *ctx, int stage);
 int wlg_contract_stats(wlg_ctx *ctx, long *launches, double *ms);
 /* the same by launch kind: the fused time-slice launches, the per-(R,T) chain
  * (or streaming) launches, the chain launch of each R >= 2 that also forms its
- * spatial transporters, and the finish launch of each R (its launch count
+ * spatial transporters, the finish launch of each R, and the fused launches
+ * over the stored FP32 transporters of contraction form 7 (its launch count
  * and summed duration, measured with HIP events around each group of launches) */
-enum { WLG_L_FUSED = 0, WLG_L_CHAIN, WLG_L_CHAIN_BUILD, WLG_L_FINISH, WLG_L_COUNT };
+enum { WLG_L_FUSED = 0, WLG_L_CHAIN, WLG_L_CHAIN_BUILD, WLG_L_FINISH, WLG_L_STORED, WLG_L_COUNT };
 int wlg_launch_stats(wlg_ctx *ctx, int kind, long *launches, double *ms);
 
 #ifdef __cplusplus

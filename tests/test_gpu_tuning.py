@@ -21,7 +21,8 @@ def test_contraction_variants_agree(wl, precision):
         lat.upload(u)
         for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
                             (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40)),
-                            (wl.KERNEL_CHAIN2, (1, 6, 512)), (wl.KERNEL_HYBRID, (1, 8, 40))):
+                            (wl.KERNEL_CHAIN2, (1, 6, 512)), (wl.KERNEL_HYBRID, (1, 8, 40)),
+                            (wl.KERNEL_STORED, (1, 8, 40))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)
@@ -33,14 +34,15 @@ def test_contraction_variants_agree(wl, precision):
 
 
 # fused time-slice forms (4: every T; 6: the FP32 blocks T <= 8 of the mixed
-# path, the one-lane chain form beyond): ragged pair blocks (spatial slice not
-# a multiple of 16), start groups that overrun N0, open and periodic
-# boundaries, and on the mixed path T blocks on both sides of the FP32/FP64
-# split at T = 9
+# path, the one-lane chain form beyond; 7: as 6, with the blocks from T = 10
+# fused over the stored transporters): ragged pair blocks (spatial slice not a
+# multiple of 16), start groups that overrun N0, open and periodic boundaries,
+# and on the mixed path T blocks on both sides of the FP32/FP64 split at T = 9
+# and stored blocks of one and of seven T
 @pytest.mark.parametrize("dims,bc,mwlt,mwlr", [((10, 5, 6, 7), 0, 5, 2), ((20, 5, 6, 7), 3, 11, 3),
-                                               ((18, 4, 4, 6), 0, 10, 2)])
+                                               ((18, 4, 4, 6), 0, 10, 2), ((36, 4, 4, 6), 3, 16, 2)])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("kernel", [4, 6])
+@pytest.mark.parametrize("kernel", [4, 6, 7])
 def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel):
     nss = [0, 2, 3]
     u = O.generate(dims, bc, 45, 0.3)
@@ -129,7 +131,7 @@ def test_two_lane_chain(wl, nss, bc, precision):
 
 def test_fused_rejects_bad_shapes(wl):
     with wl.Lattice((8, 4, 4, 4), 3) as lat:
-        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 7), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 8), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
                              (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
             with pytest.raises(wl.WloopError):
                 lat.tune(param, value)

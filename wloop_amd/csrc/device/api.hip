@@ -383,16 +383,22 @@ struct Contract {
 // mixed path the blocks of T <= 8 run in FP32, those from T = 9 in FP64, or,
 // restarted (restart = 1), in FP32 from an FP32 copy of the FP64 L_Ta, so that
 // no open loop passes through more FP32 products than those of T <= 8
+// kind: BLK_FP32 / BLK_FP64 carry the open loops through the block in that
+// precision; BLK_STORED forms them afresh at every T from the stored FP32
+// copies of the FP64-accumulated time transporters (mixed path)
+enum { BLK_FP32 = 0, BLK_FP64 = 1, BLK_STORED = 2 };
 struct DepthBlock {
-  int Ta, d, fp64;
+  int Ta, d, kind;
 };
 
 static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth, int restart) {
   std::vector<DepthBlock> b;
   const int split = prec == WLG_MIXED ? (mwlt < kTimeFp64From - 1 ? mwlt : kTimeFp64From - 1) : 0;
-  for (int T = 1; T <= split; T += depth) b.push_back({T, split - T + 1 < depth ? split - T + 1 : depth, 0});
+  for (int T = 1; T <= split; T += depth)
+    b.push_back({T, split - T + 1 < depth ? split - T + 1 : depth, BLK_FP32});
   for (int T = split + 1; T <= mwlt; T += depth)
-    b.push_back({T, mwlt - T + 1 < depth ? mwlt - T + 1 : depth, (prec == WLG_MIXED && restart) ? 0 : 1});
+    b.push_back({T, mwlt - T + 1 < depth ? mwlt - T + 1 : depth,
+                 (prec == WLG_MIXED && restart) ? BLK_FP32 : BLK_FP64});
   return b;
 }
 
@@ -400,7 +406,7 @@ static std::vector<DepthBlock> depth_blocks(int prec, int mwlt, int depth, int r
 static int depth_fp64_starts(int prec, int mwlt, int depth) {
   if (prec != WLG_MIXED) return 0;
   int n = 0;
-  for (const DepthBlock& b : depth_blocks(prec, mwlt, depth, 0)) n += b.fp64;
+  for (const DepthBlock& b : depth_blocks(prec, mwlt, depth, 0)) n += b.kind == BLK_FP64;
   return n;
 }
 
@@ -424,7 +430,7 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64,
   o.tmp = off; off += (p->op_smear == 1 ? 12 : (lean ? 3 : 9)) * sl;
   int n32 = 0, n64 = 0;  // block starts past T = 1 that need storage of their own
   for (const DepthBlock& b : blocks)
-    if (b.Ta > 1) (b.fp64 ? n64 : n32)++;
+    if (b.Ta > 1 && b.kind != BLK_STORED) (b.kind == BLK_FP64 ? n64 : n32)++;
   const bool lean_lt = lean && fused_all;  // every T in a fused block: starts only
   if (prec == WLG_MIXED) {
     o.src32 = off; off += 3 * s32;
@@ -445,14 +451,19 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64,
 static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
 
 // T blocks a contraction form runs with the fused time-slice kernel: every T
-// (form 3) or, for form 5 on the mixed path, the FP32 blocks T <= 8, the
-// remaining T running one launch per (R,T) with the chain form
+// (form 3); for forms 5 and 6 on the mixed path the FP32 blocks T <= 8, and for
+// form 6 also the blocks from T = 10 over the stored transporters; the
+// remaining T run one launch per (R,T) with the chain form (the first of them
+// forming S_R)
 static std::vector<DepthBlock> fused_blocks(const Contract& k, int prec, int mwlt) {
   std::vector<DepthBlock> b;
   if (k.kernel == 3) return depth_blocks(prec, mwlt, k.depth, k.restart);
-  if (k.kernel == 5 && prec == WLG_MIXED)
+  if ((k.kernel == 5 || k.kernel == 6) && prec == WLG_MIXED)
     for (const DepthBlock& x : depth_blocks(prec, mwlt, k.depth, 0))
-      if (!x.fp64) b.push_back(x);
+      if (x.kind == BLK_FP32) b.push_back(x);
+  if (k.kernel == 6 && prec == WLG_MIXED)
+    for (int T = kTimeFp64From + 1; T <= mwlt; T += 16)
+      b.push_back({T, mwlt - T + 1 < 16 ? mwlt - T + 1 : 16, BLK_STORED});
   return b;
 }
 
@@ -470,7 +481,7 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   k.kernel = tk > 0 ? tk - 1 : (mixed ? 5 : 4);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[6] = {8, 512, 512, 128, 512, 512};  // per form, measured at 48^3x96
+  static const int wg_default[7] = {8, 512, 512, 128, 512, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : 8;
@@ -478,7 +489,7 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
   // time links on the FP64 blocks of the mixed path)
   const int tsb = mixed ? 4 : 8;
-  while (k.starts > 4 && contract_depth_lds(c->cur_msl, k.starts, tsb, 8) > 163840) k.starts /= 2;
+  while (k.starts > 4 && contract_depth_lds(c->cur_msl, k.starts, tsb, 8, 0) > 163840) k.starts /= 2;
   return k;
 }
 
@@ -611,21 +622,27 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
   const v2<TS>* S[5];
   const v2<TS>* prev[5];
   v2<TS>* bufs[2] = {sbuf0, sbuf1};
-  // T values covered by the fused blocks
-  int tfused = 0;
-  for (const DepthBlock& b : blocks) tfused = b.Ta + b.d - 1 > tfused ? b.Ta + b.d - 1 : tfused;
-  // form 5 continues with the one-lane chain (mixed) / two-lane chain (FP64)
-  const int kc = k.kernel == 5 ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
+  // T values the fused blocks do not cover run one chain launch per (R,T)
+  std::vector<char> covered(p->mwlt + 1, 0);
+  for (const DepthBlock& b : blocks)
+    for (int T = b.Ta; T < b.Ta + b.d && T <= p->mwlt; T++) covered[T] = 1;
+  std::vector<int> tchain;
+  for (int T = 1; T <= p->mwlt; T++)
+    if (!covered[T]) tchain.push_back(T);
+  // forms 5 and 6 continue with the one-lane chain (mixed) / two-lane chain (FP64)
+  const int kc = (k.kernel == 5 || k.kernel == 6) ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
   // with the one-lane (FP32 level links) or two-lane (FP64) chain form for the
   // per-(R,T) launches, the first of them forms the spatial transporters of its
-  // R (no separate pass over F, S_{R-1} and S_R)
+  // R (no separate pass over F, S_{R-1} and S_R), ahead of every other launch
+  // of the R
   const bool build = ((sizeof(TS) == 4 && kc == 2) || (sizeof(TS) == 8 && kc == 4)) && msl <= 4 &&
-                     tfused < p->mwlt;
+                     !tchain.empty();
   for (int r = 0; r < p->mwlr; r++) {
     const int R = r + 1;
     // events per R: start, spatial transporters, chain launch forming S_R,
-    // fused launches, chain launches, finish (the timers read them in this order)
-    hipEvent_t e0, e1, eb, ef, ec, e2;
+    // fused launches, fused launches over stored transporters, chain
+    // launches, finish (the timers read them in this order)
+    hipEvent_t e0, e1, eb, ef, es, ec, e2;
     if (record(c, &e0)) return -1;
     const bool build_r = build && R > 1;
     if (R == 1) {
@@ -640,22 +657,30 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
       for (int i = 0; i < msl; i++) S[i] = out + i * lvl;
     }
     if (record(c, &e1)) return -1;
-    int tnext = tfused;  // first T of the per-(R,T) launches
+    size_t cnext = 0;  // next entry of tchain
     if (build_r) {
-      launch_contract_build<TS, TC>(L, msl, R, tnext + 1, LT[tnext], lev, prev, bufs[R % 2],
-                                    c->partial + (size_t)tnext * pstride, k.wg, kc == 4, c->st);
+      const int T = tchain[0];
+      launch_contract_build<TS, TC>(L, msl, R, T, LT[T - 1], lev, prev, bufs[R % 2],
+                                    c->partial + (size_t)(T - 1) * pstride, k.wg, kc == 4, c->st);
       CKL();
       c->contract_launches += 1;
       c->kind_launches[WLG_L_CHAIN_BUILD] += 1;
-      tnext++;
+      cnext = 1;
     }
     if (record(c, &eb)) return -1;
-    if (!blocks.empty()) {
+    // fused blocks in two groups, the blocks over stored transporters last
+    for (int group = 0; group < 2; group++) {
+      long nl = 0;
       for (size_t q = 0; q < blocks.size(); q++) {
         const DepthBlock& b = blocks[q];
+        if ((b.kind == BLK_STORED) != (group == 1)) continue;
         double* part = c->partial + (size_t)(b.Ta - 1) * pstride;
+        nl++;
         if constexpr (sizeof(TS) == 4) {
-          if (b.fp64)
+          if (b.kind == BLK_STORED)
+            launch_contract_stored(L, msl, R, b.Ta, b.d, k.starts, k.wg, (const float2*)bptr[q].lta, S,
+                                   part, (long)pstride, c->st);
+          else if (b.kind == BLK_FP64)
             launch_contract_depth<float, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                  (const double2*)bptr[q].lta,
                                                  (const double2*)bptr[q].u0, S, part,
@@ -673,17 +698,18 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
         }
         CKL();
       }
-      c->contract_launches += (long)blocks.size();
-      c->kind_launches[WLG_L_FUSED] += (long)blocks.size();
+      c->contract_launches += nl;
+      c->kind_launches[group == 0 ? WLG_L_FUSED : WLG_L_STORED] += nl;
+      if (record(c, group == 0 ? &ef : &es)) return -1;
     }
-    if (record(c, &ef)) return -1;
-    for (int t = tnext; t < p->mwlt; t++) {
-      launch_contract<TS, TC>(L, msl, R, t + 1, LT[t], S, c->partial + (size_t)t * pstride, kc,
+    for (size_t q = cnext; q < tchain.size(); q++) {
+      const int T = tchain[q];
+      launch_contract<TS, TC>(L, msl, R, T, LT[T - 1], S, c->partial + (size_t)(T - 1) * pstride, kc,
                               k.wg, c->st);
       CKL();
     }
-    c->contract_launches += p->mwlt - tnext;
-    c->kind_launches[WLG_L_CHAIN] += p->mwlt - tnext;
+    c->contract_launches += (long)(tchain.size() - cnext);
+    c->kind_launches[WLG_L_CHAIN] += (long)(tchain.size() - cnext);
     if (record(c, &ec)) return -1;
     // W of this R (normalisation and wl_idx store, wloop.c:535, :584-597)
     launch_finish_wloop(L.n[0], msl, p->mwlt, p->mwlr, r, contract_slots(L, k.kernel, k.wg),
@@ -710,7 +736,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   const Contract k = contract_choice(c, prec);
   const std::vector<DepthBlock> blocks = fused_blocks(k, prec, p->mwlt);
   int nb64 = 0;
-  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
+  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.kind == BLK_FP64;
   Layout o = wloop_layout(L, p, prec, nb64, c->tune[WLG_TUNE_LEAN] == 1, blocks, k.kernel == 3);
   if (scratch(c, o.total)) return -1;
   size_t wls = (size_t)msl * msl * p->mwlt * p->mwlr * L.n[0];
@@ -748,11 +774,11 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
     CKL();
     lts[0] = lt32;
     int m64 = 0, m32 = 0, n32 = 0;
-    for (const DepthBlock& b : blocks) n32 += !b.fp64 && b.Ta > 1;
+    for (const DepthBlock& b : blocks) n32 += b.kind == BLK_FP32 && b.Ta > 1;
     for (int T = 2; T <= tmax; T++) {
       float2* out32 = lt32 + (size_t)(T - 1) * 9 * L.V;
       bool start32 = false;  // an FP32 block starts at T
-      for (const DepthBlock& b : blocks) start32 = start32 || (!b.fp64 && b.Ta == T);
+      for (const DepthBlock& b : blocks) start32 = start32 || (b.kind == BLK_FP32 && b.Ta == T);
       if (o.lean_lt)  // [L_1][block starts][two in flight]
         out32 = lt32 + (size_t)(start32 ? 1 + m32++ : 1 + n32 + T % 2) * 9 * L.V;
       if (T < kTimeFp64From) {
@@ -761,7 +787,7 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
         lts[T - 1] = out32;
       } else {
         bool start = false;
-        for (const DepthBlock& b : blocks) start = start || (b.fp64 && b.Ta == T);
+        for (const DepthBlock& b : blocks) start = start || (b.kind == BLK_FP64 && b.Ta == T);
         double2* dst = start ? starts64 + (size_t)(m64++) * 9 * L.V : ch[T % 2];
         float2* o32 = (k.kernel == 3 && !start32) ? nullptr : out32;
         if (T == kTimeFp64From)
@@ -776,8 +802,8 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
       CKL();
     }
     for (size_t q = 0; q < blocks.size(); q++) {
-      if (blocks[q].fp64) bptr[q] = {l64[blocks[q].Ta], U};
-      else bptr[q] = {lts[blocks[q].Ta - 1], lt32};
+      if (blocks[q].kind == BLK_FP64) bptr[q] = {l64[blocks[q].Ta], U};
+      else bptr[q] = {lts[blocks[q].Ta - 1], blocks[q].kind == BLK_FP32 ? lt32 : nullptr};
     }
     if (record(c, &e_tt)) return -1;
     ev_before_rt = c->ev_used;
@@ -831,15 +857,15 @@ static int run_wloop(wlg_ctx* c, const double2* U, const wlg_parms* p, int prec,
   }
   CK(hipEventElapsedTime(&ms, e_lev, e_tt));
   c->t_ms[WLG_T_TIME_TR] += ms;
-  static const int kinds[4] = {WLG_L_CHAIN_BUILD, WLG_L_FUSED, WLG_L_CHAIN, WLG_L_FINISH};
-  for (size_t q = ev_before_rt; q + 5 < ev_after_rt; q += 6) {  // one R (see run_transport_contract)
+  static const int kinds[5] = {WLG_L_CHAIN_BUILD, WLG_L_FUSED, WLG_L_STORED, WLG_L_CHAIN, WLG_L_FINISH};
+  for (size_t q = ev_before_rt; q + 6 < ev_after_rt; q += 7) {  // one R (see run_transport_contract)
     float a = 0.f, b = 0.f;
     CK(hipEventElapsedTime(&a, c->ev[q], c->ev[q + 1]));
-    CK(hipEventElapsedTime(&b, c->ev[q + 1], c->ev[q + 5]));
+    CK(hipEventElapsedTime(&b, c->ev[q + 1], c->ev[q + 6]));
     c->t_ms[WLG_T_SPACE_TR] += a;
     c->t_ms[WLG_T_CONTRACT] += b;
     c->contract_ms += b;
-    for (int m = 0; m < 4; m++) {
+    for (int m = 0; m < 5; m++) {
       float d = 0.f;
       CK(hipEventElapsedTime(&d, c->ev[q + 1 + m], c->ev[q + 2 + m]));
       c->kind_ms[kinds[m]] += d;
@@ -1002,11 +1028,12 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
                    "them before changing a tuning parameter", c->batch_refs);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 6)
+      value > 7)
     return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
                    "per pair), 3 (chain, one lane per pair), 4 (fused time slices), 5 (chain, "
-                   "two lanes per pair) or 6 (fused time slices for the FP32 T blocks of the "
-                   "mixed path, chain form beyond)");
+                   "two lanes per pair), 6 (fused time slices for the FP32 T blocks of the "
+                   "mixed path, chain form beyond) or 7 (as 6, with the T blocks from 10 fused "
+                   "over the stored transporters)");
   if ((param == WLG_TUNE_FP64_CONTRACT_DEPTH || param == WLG_TUNE_MIXED_CONTRACT_DEPTH) &&
       value > 16)
     return set_err("time-slice depth must be 0 (default) or 1..16");
@@ -1043,7 +1070,7 @@ static int prepare_measure(wlg_ctx* c, const wlg_parms* p, int prec) {
   const Contract k = contract_choice(c, prec);
   const std::vector<DepthBlock> blocks = fused_blocks(k, prec, p->mwlt);
   int nb64 = 0;
-  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.fp64;
+  for (const DepthBlock& b : blocks) nb64 += (prec == WLG_MIXED) && b.kind == BLK_FP64;
   Layout o = wloop_layout(L, p, prec, nb64, c->tune[WLG_TUNE_LEAN] == 1, blocks, k.kernel == 3);
   size_t hyp = 24 * slot64(L);
   if (scratch(c, o.total > hyp ? o.total : hyp)) return -1;

@@ -67,11 +67,17 @@ void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT
 // fused time-slice form: T = Ta .. Ta+d-1 of one R, W start slices per
 // workgroup (one wave each), nslot workgroups per start group; partial points
 // at the block of T = Ta, pstride apart per T
-size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes);
+// LDS bytes of a fused launch (stored: the form over stored FP32 transporters)
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored);
 template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
                            double* partial, long pstride, hipStream_t st);
+// fused launch over T = Ta .. Ta+d-1 with D_i(T) formed from the stored FP32
+// transporters LTa + (T-Ta) 9V (mixed path, FP32 level links)
+void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
+                            const float2* LTa, const float2* const* S, double* partial, long pstride,
+                            hipStream_t st);
 // W of one R (index r) from the partials of its (R,T) launches
 void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);

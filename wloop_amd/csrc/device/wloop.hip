@@ -659,9 +659,17 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // b serves start group b / nslot and pair blocks b % nslot + m nslot, so the
 // workgroups that read the same pair blocks sit on the same XCD when nslot is
 // a multiple of 8.  Deterministic.  Requires msl <= 4, d <= 16.
+//
+// STORED (mixed path, T >= 10): the time transporters of the block's T are the
+// stored FP32 copies of the FP64-accumulated L_T (LTa + (T-Ta) 9V), and
+//     D_i(T) = L_T(x)^dag S_i(x) L_T(x+Rk)
+// is formed afresh at every T from them (the arithmetic of the chain forms, so
+// no time link is accumulated in FP32).  The ring then holds the sinks only;
+// each wave keeps the source links S_i(x) of its start slice in registers and
+// its own L_T at x and x+Rk in a two-slot LDS buffer, loaded one step ahead.
 // waves per SIMD the LDS of the form allows at W = 4 (FP64 time links) or
 // W = 4 / 8 (FP32 time links), as register target
-template <int MSL, int W, class TS, class TC>
+template <int MSL, int W, class TS, class TC, bool STORED>
 __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) void contract_depth(Lat L, int R, int Ta, int d, int nslot,
                                                          const v2<TC>* __restrict__ LTa,
                                                          const v2<TC>* __restrict__ U0,
@@ -673,12 +681,14 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
   constexpr int NT = 64 * W;
   constexpr int NS = MSL * 9 * 16;             // sink 2-vectors of a slot: [j][e][p]
   constexpr int NU = 2 * 9 * 16;               // time-link 2-vectors: [x / x+Rk][e][p]
-  constexpr int PS = (NS + NT - 1) / NT, PU = (NU + NT - 1) / NT;
-  constexpr int SLOT = NS * (int)sizeof(v2<TS>) + NU * (int)sizeof(v2<TC>);  // bytes
+  constexpr int PS = (NS + NT - 1) / NT, PU = STORED ? 0 : (NU + NT - 1) / NT;
+  constexpr int PL = STORED ? (NU + 63) / 64 : 0;  // L_T 2-vectors per lane (own wave)
+  constexpr int SLOT = NS * (int)sizeof(v2<TS>) + (STORED ? 0 : NU * (int)sizeof(v2<TC>));  // bytes
   extern __shared__ double lds_d[];
   double* acc = lds_d;                                   // [W][16][16]
   long* xr_tab = (long*)(lds_d + W * 256);               // [16] x+Rk of the block's pairs
   char* ring = (char*)(xr_tab + 16);                     // [W+1] slots
+  v2<TC>* ltb = (v2<TC>*)(ring + (W + 1) * SLOT);        // STORED: [W][2][NU]
   const int N0 = L.n[0];
   const long slice = L.st[0];
   const long V = L.V;
@@ -693,8 +703,9 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
 
   // this thread's elements of a ring slot: sink element q = (j*9 + e)*16 + pp
   // and time-link element q = (h*9 + e)*16 + pp (h = 1: the site x + R k)
-  int sj_[PS], sp_[PS], uh_[PU], up_[PU];
-  long uo_[PU];
+  constexpr int PU1 = PU > 0 ? PU : 1;
+  int sj_[PS], sp_[PS], uh_[PU1], up_[PU1];
+  long uo_[PU1];
   const v2<TS>* sb_[PS];  // level plane of the element, at the element's pair
 #pragma unroll
   for (int m = 0; m < PS; m++) {
@@ -734,9 +745,9 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     // loads sit in no branch and are waited for only at the store, one step
     // later.
     v2<TS> ps[PS];
-    v2<TC> pu[PU];
-    long so_[PS], uoff_[PU];
-    bool sok_[PS], uok_[PU];
+    v2<TC> pu[PU1];
+    long so_[PS], uoff_[PU1];
+    bool sok_[PS], uok_[PU1];
     auto offsets = [&]() {
 #pragma unroll
       for (int m = 0; m < PS; m++) {
@@ -781,6 +792,58 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     if (threadIdx.x < 16) xr_tab[threadIdx.x] = xr;
     __syncthreads();  // xr_tab; the previous block's last reads of the ring
     offsets();
+    // STORED: this wave's L_T elements q = lane + 64 m = (h*9 + e)*16 + pp at
+    // its start slice, loaded unconditionally (element 0 of the plane where
+    // there is none) and zeroed at the store
+    v2<TC> pl[PL > 0 ? PL : 1];
+    long lo_[PL > 0 ? PL : 1];
+    bool lok_[PL > 0 ? PL : 1];
+    M3<TC> Msrc;
+    if constexpr (STORED) {
+      const long xslice = (long)(wave_live ? x0 : 0) * slice;
+#pragma unroll
+      for (int m = 0; m < PL; m++) {
+        const int q = lane + 64 * m;
+        const int hh = q / 144, e = (q >> 4) % 9, pp = q & 15;
+        lok_[m] = q < NU && xs0 + pp < slice;
+        lo_[m] = lok_[m] ? (long)e * V + xslice + (hh ? xr_tab[pp] : xs0 + pp) : 0;
+      }
+      set_zero(Msrc);
+      if (live) {
+        M3<TS> t;
+        load(t, level_ptr(S, i), V, k, xslice + xs);
+        Msrc = convert<TC>(t);
+      }
+    }
+    auto lt_fetch = [&](int n) {  // L_{Ta+n} of this wave's pairs into registers
+#pragma unroll
+      for (int m = 0; m < PL; m++) pl[m] = LTa[(long)n * 9 * V + lo_[m]];
+    };
+    auto lt_put = [&](int b) {  // from registers into this wave's buffer b
+#pragma unroll
+      for (int m = 0; m < PL; m++) {
+        const int q = lane + 64 * m;
+        v2<TC> v;
+        v.x = lok_[m] ? pl[m].x : TC(0);
+        v.y = lok_[m] ? pl[m].y : TC(0);
+        if (q < NU) ltb[(w * 2 + b) * NU + q] = v;
+      }
+    };
+    M3<TC> D;
+    set_zero(D);
+    if constexpr (STORED) {
+      // sinks of the slices x0g+Ta+o in ring slots o = 0 .. W-1; L_Ta into
+      // buffer 0, L_{Ta+1} into registers
+#pragma unroll FILL
+      for (int o = 0; o < W; o++) {
+        fetch(o);
+        put(o);
+      }
+      lt_fetch(0);
+      lt_put(0);
+      if (d > 1) lt_fetch(1);
+      __syncthreads();
+    } else {
     // the start slices x0g+w with S_i(x) and L_Ta at x and x + R k: for Ta = 1
     // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
     // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
@@ -799,8 +862,6 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     }
     __syncthreads();
     // D_i(Ta) = L_Ta(x)^dag S_i(x) L_Ta(x + R k)
-    M3<TC> D;
-    set_zero(D);
     if (live) {
       const char* slot = ring + (Ta == 1 ? (w + W) % (W + 1) : w) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
@@ -834,6 +895,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       }
       __syncthreads();
     }
+    }  // not STORED
 
     // the slice of ring offset n+W (read by wave W-1 at step n+1) is fetched
     // into registers during step n-1 and stored at the top of step n into the
@@ -848,6 +910,25 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
       const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
+      if constexpr (STORED) {
+        // D_i(T) = L_T(x)^dag S_i(x) L_T(x + R k) from this wave's buffer
+        const v2<TC>* lb = ltb + (w * 2 + (n & 1)) * NU + p;
+        M3<TC> a;
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          v2<TC> c = lb[e * 16];
+          a.re[e] = c.x;
+          a.im[e] = c.y;
+        }
+        const M3<TC> h = dag_mul(a, Msrc);
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+          v2<TC> c = lb[(9 + e) * 16];
+          a.re[e] = c.x;
+          a.im[e] = c.y;
+        }
+        D = mul(h, a);
+      }
       // traces against the sinks S_j(x + (Ta+n) 0): products in the compute
       // type, each trace converted to FP64 before it is summed
       double tr[4];
@@ -879,7 +960,12 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       v += row_dpp<kQuadXor1>(v);
       if (wave_live && i < MSL && (p & 3) == 0 && (p >> 2) < MSL)
         acc[(w * 16 + n) * 16 + i * MSL + (p >> 2)] += v;
-      if (more) {
+      if constexpr (STORED) {
+        if (more) {
+          lt_put((n + 1) & 1);
+          if (n + 2 < d) lt_fetch(n + 2);
+        }
+      } else if (more) {
         if (live) {
           M3<TC> u;
 #pragma unroll
@@ -1110,29 +1196,43 @@ void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT
   }
 }
 
-size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes) {
-  const size_t slot = (size_t)msl * 9 * 16 * 2 * ts_bytes + (size_t)2 * 9 * 16 * 2 * tc_bytes;
-  return (size_t)W * 256 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot;
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored) {
+  const size_t nu = (size_t)2 * 9 * 16 * 2 * tc_bytes;  // time links (or L_T) of a slice
+  const size_t slot = (size_t)msl * 9 * 16 * 2 * ts_bytes + (stored ? 0 : nu);
+  return (size_t)W * 256 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot +
+         (stored ? (size_t)W * 2 * nu : 0);
 }
 
-template <int MSL, int W, class TS, class TC>
+template <int MSL, int W, class TS, class TC, bool STORED>
 static void depth_launch(const Lat& L, int R, int Ta, int d, int nslot, const v2<TC>* LTa,
                          const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
                          hipStream_t st) {
-  const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC));
+  const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC), STORED);
   const dim3 grid((unsigned)(((L.n[0] + W - 1) / W) * nslot));
-  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC>), grid, dim3(64 * W), lds, st, L, R, Ta, d,
-                     nslot, LTa, U0, sp, partial, pstride);
+  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED>), grid, dim3(64 * W), lds, st, L, R,
+                     Ta, d, nslot, LTa, U0, sp, partial, pstride);
 }
 
-template <int MSL, class TS, class TC>
+template <int MSL, class TS, class TC, bool STORED>
 static void depth_launch_w(const Lat& L, int R, int Ta, int d, int W, int nslot, const v2<TC>* LTa,
                            const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
                            hipStream_t st) {
   if (W >= 8)
-    depth_launch<MSL, 8, TS, TC>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 8, TS, TC, STORED>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
   else
-    depth_launch<MSL, 4, TS, TC>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 4, TS, TC, STORED>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+}
+
+template <class TS, class TC, bool STORED>
+static void depth_launch_msl(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
+                             const v2<TC>* LTa, const v2<TC>* U0, const LevelPtrs<TS>& sp,
+                             double* partial, long pstride, hipStream_t st) {
+  switch (msl) {
+    case 1: depth_launch_w<1, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 2: depth_launch_w<2, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 3: depth_launch_w<3, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    default: depth_launch_w<4, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+  }
 }
 
 template <class TS, class TC>
@@ -1141,12 +1241,16 @@ void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, i
                            double* partial, long pstride, hipStream_t st) {
   LevelPtrs<TS> sp;
   for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
-  switch (msl) {
-    case 1: depth_launch_w<1, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 2: depth_launch_w<2, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 3: depth_launch_w<3, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    default: depth_launch_w<4, TS, TC>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-  }
+  depth_launch_msl<TS, TC, false>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
+}
+
+void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
+                            const float2* LTa, const float2* const* S, double* partial, long pstride,
+                            hipStream_t st) {
+  LevelPtrs<float> sp;
+  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
+  depth_launch_msl<float, float, true>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial, pstride,
+                                       st);
 }
 
 // kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl

@@ -62,7 +62,9 @@ def kernel_key(name):
     if m:
         return f"hyp{m.group(1)}/" + ("mixed" if a[0] == "float" else "fp64")
     if re.search(r"\bcontract_depth<", name):
-        ts, tc = a[-2], a[-1]
+        ts, tc = a[2], a[3]
+        if len(a) > 4 and a[4] == "true":
+            return "stored/mixed"
         if ts == "double":
             return "fused/fp64"
         return "fused/mixed" if tc == "float" else "fused64/mixed"
