@@ -93,8 +93,8 @@ def ape_flop_per_site(p):
 
 
 KERNEL_LABELS = {
-    ("ape_step", "mixed"): "ape_step<float, float>",
-    ("ape_step", "fp64"): "ape_step<double, double>",
+    ("ape_step", "mixed"): "ape_step<float, float, true> (packed FP32 links)",
+    ("ape_step", "fp64"): "ape_step<double, double, false>",
     ("hyp", "mixed"): "hyp_level1/2/3<float, double, float> (3 launches)",
     ("hyp", "fp64"): "hyp_level1/2/3<double, double, double> (3 launches)",
     ("fused", "mixed"): "contract_depth<4, {W}, float, float, false> (fused time slices, FP32 blocks)",

@@ -146,7 +146,10 @@ enum {
    WLG_TUNE_MIXED_CONTRACT_DEPTH, WLG_TUNE_MIXED_CONTRACT_STARTS,
    WLG_TUNE_APE_FORM,   /* APE kernels: 1 one thread per link, 2 per-link staple sums then
                            lincomb and projection, 3 one thread per site, 4 per-site staple
-                           sums then lincomb and projection */
+                           sums then lincomb and projection, 5 one thread per link on FP32
+                           links packed two elements per 16 bytes (mixed path, default
+                           layout; otherwise form 1).  Default: 5 on the mixed path, 1 on
+                           the FP64 path */
    WLG_TUNE_LEAN,       /* 1: least device memory (wlg_footprint_lean): fused time-slice
                            contraction for every T, time transporters kept only at the
                            starts of its T blocks, APE steps through the level buffers */

@@ -168,13 +168,33 @@ def test_ape_forms_agree(wl, precision):
     outs = {}
     with wl.Lattice(dims, bc) as lat:
         lat.upload(u)
-        for form in (1, 2, 3, 4):
+        for form in (1, 2, 3, 4, 5):
             lat.tune(wl.TUNE_APE_FORM, form)
             outs[form] = lat.measure(p, precision)
     assert np.array_equal(outs[1], outs[2])
     assert np.array_equal(outs[3], outs[4])
+    assert np.array_equal(outs[1], outs[5])  # packed FP32 layout (FP64: form 1 itself)
     for form in (1, 3):
         assert rel_err(outs[form], ref) < (1e-12 if precision == 0 else 1e-6), form
+
+
+# APE on the packed FP32 layout: an odd volume (the packed buffers are aligned
+# inside the scratch slots), a first level that is smeared (packed from the
+# field) and levels of one step
+@pytest.mark.parametrize("nss", [[1, 2, 1], [0, 1, 3]])
+def test_ape_packed_layout(wl, nss):
+    dims, bc = (9, 5, 5, 7), 3
+    u = O.generate(dims, bc, 48, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, 3, 2))
+    p = wl.make_parms(nss, 3, 2)
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        lat.tune(wl.TUNE_APE_FORM, 1)
+        w1 = lat.measure(p, wl.MIXED)
+        lat.tune(wl.TUNE_APE_FORM, 5)
+        w5 = lat.measure(p, wl.MIXED)
+    assert np.array_equal(w1, w5)
+    assert rel_err(w5, ref) < 1e-6
 
 
 # lean layout (least device memory): only storage moves, so it gives the bits of

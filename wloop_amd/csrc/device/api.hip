@@ -528,11 +528,18 @@ extern "C" size_t wlg_footprint_lean(const int dims[4], const wlg_parms* p, int 
 // APE kernel form of the operator levels: 1 one thread per link, 2 per-link
 // staple sums then lincomb/projection, 3 one thread per site for its three
 // links, 4 per-site staple sums then lincomb/projection
+// (default: 5 on the mixed path, measured 240 against 251 ms for the 96 steps
+// at 48^3x96, 1 on the FP64 path; the lean layout has no scratch slots for
+// forms 2, 4 and 5)
 static int ape_form(const wlg_ctx* c, int prec) {
-  (void)prec;
-  const int t = c->tune[WLG_TUNE_APE_FORM];
-  if (c->tune[WLG_TUNE_LEAN] == 1 && (t == 2 || t == 4)) return t - 1;  // no staple-sum slots
-  return t > 0 ? t : 1;
+  int t = c->tune[WLG_TUNE_APE_FORM];
+  if (t == 0) t = (prec == WLG_MIXED) ? 5 : 1;
+  if (t == 5 && prec != WLG_MIXED) t = 1;  // the packed layout is for FP32 links
+  if (c->tune[WLG_TUNE_LEAN] == 1) {
+    if (t == 2 || t == 4) t -= 1;
+    if (t == 5) t = 1;
+  }
+  return t;
 }
 
 // operator levels (smear.c:99-121): F_0 = spatial links of U, then nss[sl]
@@ -551,6 +558,44 @@ static int run_levels_t(wlg_ctx* c, const v2<TS>* src, const wlg_parms* p, const
   v2<TS>* tmp0 = (v2<TS>*)tmpbase;
   v2<TS>* tmp1 = tmp0 + lvl;
   v2<TS>* tmpD = tmp0 + 2 * lvl;  // 3-d HYP intermediate (6 slots) / APE staple sums (3)
+  if constexpr (sizeof(TS) == 4) {
+    // APE form 5: the steps run on the packed FP32 layout in two 3-slot
+    // buffers in the scratch slots (2 x 15 V float4 of the 9 x 9 V float2,
+    // 256-byte aligned); each level is unpacked into its own storage.  The
+    // links and the arithmetic are those of form 1, so the levels have the
+    // same bits.
+    if (p->op_smear == 0 && !o.lean && ape_form(c, WLG_MIXED) == 5) {
+      const int proj = p->proj_iter > 0;
+      float4* pk = (float4*)(((uintptr_t)tmpbase + 255) & ~(uintptr_t)255);
+      float4* P[2] = {pk, pk + 15 * L.V};
+      const float4* cur = nullptr;
+      int which = 0;
+      for (int sl = 0; sl < msl; sl++) {
+        const int n = p->nss[sl];
+        if (n == 0) {
+          lev[sl] = (sl == 0) ? src : lev[sl - 1];
+          continue;
+        }
+        if (!cur) {
+          launch_pack_links(L.V, 3, (const float2*)((sl == 0) ? src : lev[sl - 1]), P[0], c->st);
+          CKL();
+          cur = P[0];
+          which = 1;
+        }
+        for (int step = 0; step < n; step++) {
+          launch_ape_packed(L, cur, P[which], p->alpha_ape, proj, p->proj_iter, c->st);
+          CKL();
+          cur = P[which];
+          which ^= 1;
+        }
+        v2<TS>* mine = levbuf + (sl - o.lev_alias) * lvl;
+        launch_unpack_links(L.V, 3, cur, (float2*)mine, c->st);
+        CKL();
+        lev[sl] = mine;
+      }
+      return 0;
+    }
+  }
   for (int sl = 0; sl < msl; sl++) {
     const v2<TS>* cur = (sl == 0) ? src : lev[sl - 1];
     int n = p->nss[sl];
@@ -1040,10 +1085,10 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
   if ((param == WLG_TUNE_FP64_CONTRACT_STARTS || param == WLG_TUNE_MIXED_CONTRACT_STARTS) &&
       value != 0 && value != 4 && value != 8)
     return set_err("start slices per workgroup must be 0 (default), 4 or 8");
-  if (param == WLG_TUNE_APE_FORM && value > 4)
+  if (param == WLG_TUNE_APE_FORM && value > 5)
     return set_err("APE form must be 0 (default), 1 (one thread per link), 2 (staples per link, "
-                   "then projection), 3 (one thread per site) or 4 (staples per site, then "
-                   "projection)");
+                   "then projection), 3 (one thread per site), 4 (staples per site, then "
+                   "projection) or 5 (one thread per link on packed FP32 links, mixed path)");
   if (param == WLG_TUNE_LEAN && value > 1)
     return set_err("lean layout must be 0 (default layout) or 1 (least device memory)");
   if (param == WLG_TUNE_MIXED_RESTART && value > 1)

@@ -67,6 +67,12 @@ void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT
 // fused time-slice form: T = Ta .. Ta+d-1 of one R, W start slices per
 // workgroup (one wave each), nslot workgroups per start group; partial points
 // at the block of T = Ta, pstride apart per T
+// APE step on links in the packed FP32 layout (5 float4 planes per slot:
+// elements 2q, 2q+1 in plane 5l+q), and the conversions to and from it
+void launch_ape_packed(const Lat& L, const float4* in, float4* out, double alpha, int proj, int piter,
+                       hipStream_t st);
+void launch_pack_links(long V, int nslot, const float2* src, float4* dst, hipStream_t st);
+void launch_unpack_links(long V, int nslot, const float4* src, float2* dst, hipStream_t st);
 // LDS bytes of a fused launch (stored: the form over stored FP32 transporters)
 size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored);
 template <class TS, class TC>

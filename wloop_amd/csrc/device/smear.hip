@@ -28,38 +28,39 @@ __device__ __forceinline__ site_t stencil_site(int ny, int& y) {
 }
 
 // v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
-// ln: slot of the nu-links in A, lm: slot of the mu-links in B.
-template <class TC, class TS>
+// ln: slot of the nu-links in A, lm: slot of the mu-links in B.  PK: A and B
+// in the packed FP32 layout.
+template <class TC, class TS, bool PK = false>
 __device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, site_t s, const int x[4],
                                            const v2<TS>* __restrict__ A, int ln,
                                            const v2<TS>* __restrict__ B, int lm, int mu, int nu) {
   M3<TS> a, b, c;
   const site_t xpn = fwd(L, s, x, nu, 1);
   const site_t xpm = fwd(L, s, x, mu, 1);
-  load_u(b, B, L.V, lm, xpn);
-  load_u(c, A, L.V, ln, xpm);
-  load_u(a, A, L.V, ln, s);
+  ld_link<PK>(b, B, L.V, lm, xpn);
+  ld_link<PK>(c, A, L.V, ln, xpm);
+  ld_link<PK>(a, A, L.V, ln, s);
   M3<TC> t = mul_dag(convert<TC>(b), convert<TC>(c));
   mul_acc(v, convert<TC>(a), t);
   const site_t xmn = bwd(L, s, x, nu);
   int y[4] = {x[0], x[1], x[2], x[3]};
   y[nu] = (x[nu] == 0) ? L.n[nu] - 1 : x[nu] - 1;
   const site_t xmnpm = fwd(L, xmn, y, mu, 1);
-  load_u(b, B, L.V, lm, xmn);
-  load_u(c, A, L.V, ln, xmnpm);
-  load_u(a, A, L.V, ln, xmn);
+  ld_link<PK>(b, B, L.V, lm, xmn);
+  ld_link<PK>(c, A, L.V, ln, xmnpm);
+  ld_link<PK>(a, A, L.V, ln, xmn);
   t = mul(convert<TC>(b), convert<TC>(c));
   dag_mul_acc(v, convert<TC>(a), t);
 }
 
 // out = P((1-a)U + b S), S in TC, U in TS, out in TO (TS unless given),
-// combination in FP64
-template <class TC, class TS, class TO = TS>
+// combination in FP64; PK: U and out in the packed FP32 layout
+template <class TC, class TS, class TO = TS, bool PK = false>
 __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __restrict__ U, int lu,
                                                v2<TO>* __restrict__ out, int lo, long V, site_t s,
                                                double alpha, double beta, int proj, int piter) {
   M3<TS> u;
-  load_u(u, U, V, lu, s);
+  ld_link<PK>(u, U, V, lu, s);
   M3<double> w;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
@@ -67,7 +68,7 @@ __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __
     w.im[e] = alpha * (double)u.im[e] + beta * (double)S.im[e];
   }
   if (proj) project(w, piter);
-  store_u(convert<TO>(w), out, V, lo, s);
+  st_link<PK>(convert<TO>(w), out, V, lo, s);
 }
 
 __device__ __forceinline__ int part4(int a, int b) {
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
 // in and out are 3-slot spatial fields (slot k = mu-1); Jacobi update.
 // register targets (waves per SIMD without spills): 4 with FP32 links, 3 with
 // FP64 links and FP32 staples, 2 in FP64
-template <class TC, class TS>
+// PK: in and out in the packed FP32 layout (5 float4 planes per slot)
+template <class TC, class TS, bool PK = false>
 __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2)) void ape_step(Lat L, const v2<TS>* __restrict__ in,
                                                 v2<TS>* __restrict__ out, double alpha, int proj,
                                                 int piter) {
@@ -217,9 +219,39 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2
   set_zero(S);
   for (int eta = 1; eta < 4; eta++) {
     if (eta == mu) continue;
-    staple_add<TC, TS>(S, L, s, x, in, eta - 1, in, mu - 1, mu, eta);
+    staple_add<TC, TS, PK>(S, L, s, x, in, eta - 1, in, mu - 1, mu, eta);
   }
-  decorate_store<TC, TS>(S, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
+  decorate_store<TC, TS, TS, PK>(S, in, mu - 1, out, mu - 1, L.V, s, 1.0, alpha, proj, piter);
+}
+
+// standard <-> packed FP32 layout of nslot link slots, one thread per (slot,
+// site); the values are copied unchanged
+__global__ __launch_bounds__(256) void pack_links(long V, int nslot, const float2* __restrict__ src,
+                                                  float4* __restrict__ dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nslot * V) return;
+  const int l = (int)(i / V);
+  const long s = i - (long)l * V;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const float2 a = src[(long)(9 * l + 2 * q) * V + s];
+    const float2 b = q < 4 ? src[(long)(9 * l + 2 * q + 1) * V + s] : make_float2(0.f, 0.f);
+    dst[(long)(5 * l + q) * V + s] = make_float4(a.x, a.y, b.x, b.y);
+  }
+}
+
+__global__ __launch_bounds__(256) void unpack_links(long V, int nslot, const float4* __restrict__ src,
+                                                    float2* __restrict__ dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nslot * V) return;
+  const int l = (int)(i / V);
+  const long s = i - (long)l * V;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const float4 v = src[(long)(5 * l + q) * V + s];
+    dst[(long)(9 * l + 2 * q) * V + s] = make_float2(v.x, v.y);
+    if (q < 4) dst[(long)(9 * l + 2 * q + 1) * V + s] = make_float2(v.z, v.w);
+  }
 }
 
 // ---- APE step, one thread per site -------------------------------------------
@@ -423,6 +455,22 @@ void launch_ape(const Lat& L, const v2<TS>* in, v2<TS>* out, double alpha, int p
                 hipStream_t st) {
   hipLaunchKernelGGL((ape_step<TC, TS>), grid_for(L.V, 3), dim3(256), 0, st, L, in, out, alpha,
                      proj, piter);
+}
+
+void launch_ape_packed(const Lat& L, const float4* in, float4* out, double alpha, int proj, int piter,
+                       hipStream_t st) {
+  hipLaunchKernelGGL((ape_step<float, float, true>), grid_for(L.V, 3), dim3(256), 0, st, L,
+                     (const float2*)in, (float2*)out, alpha, proj, piter);
+}
+
+void launch_pack_links(long V, int nslot, const float2* src, float4* dst, hipStream_t st) {
+  hipLaunchKernelGGL(pack_links, dim3((unsigned)(((long)nslot * V + 255) / 256)), dim3(256), 0, st, V,
+                     nslot, src, dst);
+}
+
+void launch_unpack_links(long V, int nslot, const float4* src, float2* dst, hipStream_t st) {
+  hipLaunchKernelGGL(unpack_links, dim3((unsigned)(((long)nslot * V + 255) / 256)), dim3(256), 0, st,
+                     V, nslot, src, dst);
 }
 
 template <class TC, class TS>

@@ -113,6 +113,71 @@ __device__ __forceinline__ void store_u(const M3<R>& m, v2<R>* __restrict__ f, l
   }
 }
 
+// Packed FP32 link storage (the mixed path's APE steps): elements 2q and 2q+1
+// of slot l in one float4 of plane 5l + q (element 8 with a zero pair), so a
+// link is 5 loads of 16 B per lane instead of 9 of 8 B.  The vector memory
+// address unit, which bounds the APE step, spends a fixed cost per load
+// instruction besides its cost per byte.
+__device__ __forceinline__ float4 ld4_u(const float4* base, site_t s) {
+  typedef float quad __attribute__((ext_vector_type(4)));
+  gptr<const char> b = (gptr<const char>)uniform_ptr(base);
+  const quad t = *(gptr<const quad>)(b + (uint32_t)(s * (uint32_t)sizeof(quad)));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+
+__device__ __forceinline__ void st4_u(float4* base, site_t s, float4 v) {
+  typedef float quad __attribute__((ext_vector_type(4)));
+  gptr<char> b = (gptr<char>)uniform_ptr(base);
+  quad t;
+  t.x = v.x;
+  t.y = v.y;
+  t.z = v.z;
+  t.w = v.w;
+  *(gptr<quad>)(b + (uint32_t)(s * (uint32_t)sizeof(quad))) = t;
+}
+
+__device__ __forceinline__ void load_pk_u(M3<float>& m, const float4* __restrict__ f, long V, int l,
+                                          site_t s) {
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const float4 v = ld4_u(f + (long)(5 * l + q) * V, s);
+    m.re[2 * q] = v.x;
+    m.im[2 * q] = v.y;
+    if (q < 4) {
+      m.re[2 * q + 1] = v.z;
+      m.im[2 * q + 1] = v.w;
+    }
+  }
+}
+
+__device__ __forceinline__ void store_pk_u(const M3<float>& m, float4* __restrict__ f, long V, int l,
+                                           site_t s) {
+#pragma unroll
+  for (int q = 0; q < 5; q++)
+    st4_u(f + (long)(5 * l + q) * V, s,
+          make_float4(m.re[2 * q], m.im[2 * q], q < 4 ? m.re[2 * q + 1] : 0.f,
+                      q < 4 ? m.im[2 * q + 1] : 0.f));
+}
+
+// link load / store of a field in the standard layout (PK = false) or, for
+// FP32 links, in the packed layout (PK = true; the pointer then addresses
+// float4 planes)
+template <bool PK, class R>
+__device__ __forceinline__ void ld_link(M3<R>& m, const v2<R>* __restrict__ f, long V, int l, site_t s) {
+  if constexpr (PK)
+    load_pk_u(m, (const float4*)f, V, l, s);
+  else
+    load_u(m, f, V, l, s);
+}
+
+template <bool PK, class R>
+__device__ __forceinline__ void st_link(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, site_t s) {
+  if constexpr (PK)
+    store_pk_u(m, (float4*)f, V, l, s);
+  else
+    store_u(m, f, V, l, s);
+}
+
 template <class R>
 __device__ __forceinline__ void store(const M3<R>& m, v2<R>* __restrict__ f, long V, int l, site_t s) {
 #pragma unroll
