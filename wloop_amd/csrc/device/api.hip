@@ -476,15 +476,16 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
   // defaults (measured at 48^3x96): two-lane chain form for FP64; on the
-  // mixed path the fused form for the FP32 blocks T <= 8 and the one-lane
-  // chain form beyond
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 5 : 4);
+  // mixed path form 7 (fused FP32 blocks for T <= 8, the one-lane chain at
+  // T = 9 forming S_R, the fused form over the stored transporters beyond)
+  // with 4 start slices per workgroup
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 6 : 4);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
   static const int wg_default[7] = {8, 512, 512, 128, 512, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
-  k.starts = ts > 0 ? ts : 8;
+  k.starts = ts > 0 ? ts : (mixed ? 4 : 8);
   k.restart = mixed && t[WLG_TUNE_MIXED_RESTART] == 1;
   // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
   // time links on the FP64 blocks of the mixed path)
