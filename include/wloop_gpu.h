@@ -159,6 +159,9 @@ enum {
                            the FP64 time transporter at its first T, so the time links past
                            that T are accumulated in FP32 inside the open loops (0, the
                            default: those blocks carry the open loops in FP64) */
+   WLG_TUNE_FUSED_TRACES, /* fused time-slice launches: 1 takes the msl^2 traces of a step
+                           on the vector units, 2 (the default) on the matrix cores (4x4
+                           MFMA outer products per pair, pair sums in FP64) */
    WLG_TUNE_COUNT
 };
 int wlg_tune(wlg_ctx *ctx, int param, int value);

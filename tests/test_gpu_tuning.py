@@ -43,7 +43,8 @@ def test_contraction_variants_agree(wl, precision):
                                                ((18, 4, 4, 6), 0, 10, 2), ((36, 4, 4, 6), 3, 16, 2)])
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("kernel", [4, 6, 7])
-def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel):
+@pytest.mark.parametrize("traces", [1, 2])
+def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel, traces):
     nss = [0, 2, 3]
     u = O.generate(dims, bc, 45, 0.3)
     ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
@@ -58,15 +59,45 @@ def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel):
         base = lat.measure(p, precision)  # library default form
         assert rel_err(base, ref) < tol
         lat.tune(pk, kernel)
+        lat.tune(wl.TUNE_FUSED_TRACES, traces)
         for depth, starts, wg in ((16, 4, 8), (3, 4, 5), (1, 8, 1), (8, 8, 16)):
             lat.tune(pd, depth)
             lat.tune(ps, starts)
             lat.tune(pw, wg)
-            print(f"[fused] kernel={kernel} dims={dims} bc={bc} prec={precision} depth={depth} "
-                  f"starts={starts} wg={wg}", flush=True)
+            print(f"[fused] kernel={kernel} traces={traces} dims={dims} bc={bc} prec={precision} "
+                  f"depth={depth} starts={starts} wg={wg}", flush=True)
             w = lat.measure(p, precision)
             assert rel_err(w, ref) < tol, (depth, starts, wg)
             assert rel_err(w, base) < (1e-13 if precision == 0 else 1e-7), (depth, starts, wg)
+
+
+# traces of the fused launches on the matrix cores (4x4 outer products per
+# pair) against those on the vector units, for every level count the fused
+# forms take (levels i >= msl of a pair's four lanes are idle) and an odd
+# slice (ragged last pair block)
+@pytest.mark.parametrize("nss", [[0], [0, 2], [0, 1, 2], [1, 1, 2, 3]])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kernel", [4, 7])
+def test_fused_traces_matrix_cores(wl, nss, precision, kernel):
+    dims, bc, mwlt, mwlr = (26, 5, 6, 7), 3, 12, 3
+    u = O.generate(dims, bc, 47, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    pk = wl.TUNE_MIXED_CONTRACT_KERNEL if precision else wl.TUNE_FP64_CONTRACT_KERNEL
+    tol = 1e-12 if precision == 0 else 1e-6
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        lat.tune(pk, kernel)
+        lat.tune(wl.TUNE_FUSED_TRACES, 1)
+        vec = lat.measure(p, precision)
+        lat.tune(wl.TUNE_FUSED_TRACES, 2)
+        print(f"[traces] kernel={kernel} nss={nss} prec={precision}", flush=True)
+        mat = lat.measure(p, precision)
+        again = lat.measure(p, precision)
+    assert rel_err(vec, ref) < tol
+    assert rel_err(mat, ref) < tol
+    assert rel_err(mat, vec) < (1e-13 if precision == 0 else 1e-7)
+    assert np.array_equal(mat, again)
 
 
 # mixed path, fused form for every T with the blocks from T = 9 in FP32, each
@@ -132,6 +163,7 @@ def test_two_lane_chain(wl, nss, bc, precision):
 def test_fused_rejects_bad_shapes(wl):
     with wl.Lattice((8, 4, 4, 4), 3) as lat:
         for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 8), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+                             (wl.TUNE_FUSED_TRACES, 3),
                              (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
             with pytest.raises(wl.WloopError):
                 lat.tune(param, value)

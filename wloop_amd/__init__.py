@@ -19,7 +19,7 @@ OPEN, PERIODIC = 0, 3
 T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
 (TUNE_FP64_CONTRACT_KERNEL, TUNE_FP64_CONTRACT_WG, TUNE_MIXED_CONTRACT_KERNEL, TUNE_MIXED_CONTRACT_WG,
  TUNE_FP64_CONTRACT_DEPTH, TUNE_FP64_CONTRACT_STARTS, TUNE_MIXED_CONTRACT_DEPTH,
- TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM, TUNE_LEAN, TUNE_MIXED_RESTART) = range(11)
+ TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM, TUNE_LEAN, TUNE_MIXED_RESTART, TUNE_FUSED_TRACES) = range(12)
 (KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH, KERNEL_CHAIN2, KERNEL_HYBRID,
  KERNEL_STORED) = 1, 2, 3, 4, 5, 6, 7
 KERNEL_NAMES = {KERNEL_STREAM: "streaming", KERNEL_CHAIN: "chain (msl lanes per pair)",

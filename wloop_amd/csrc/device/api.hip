@@ -377,6 +377,7 @@ struct Layout {
 // fused time-slice form, the T depth per launch and start slices per workgroup
 struct Contract {
   int kernel, wg, depth, starts, restart;
+  int mx;  // fused launches: traces on the matrix cores
 };
 
 // T blocks of the fused time-slice form: [Ta, Ta+d-1] covering 1..mwlt; on the
@@ -487,10 +488,14 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : (mixed ? 4 : 8);
   k.restart = mixed && t[WLG_TUNE_MIXED_RESTART] == 1;
+  // traces of the fused launches on the matrix cores unless asked otherwise
+  // (48^3x96: FP64 fused launch 31.4 against 36.8 ms; FP32 the same time and
+  // the same bits)
+  k.mx = t[WLG_TUNE_FUSED_TRACES] != 1;
   // LDS of the fused form: at most 160 KiB per workgroup on gfx950 (FP64
   // time links on the FP64 blocks of the mixed path)
   const int tsb = mixed ? 4 : 8;
-  while (k.starts > 4 && contract_depth_lds(c->cur_msl, k.starts, tsb, 8, 0) > 163840) k.starts /= 2;
+  while (k.starts > 4 && contract_depth_lds(c->cur_msl, k.starts, tsb, 8, 0, 16) > 163840) k.starts /= 2;
   return k;
 }
 
@@ -508,7 +513,7 @@ static size_t footprint(const int dims[4], const wlg_parms* p, int prec, int lea
   // x0: the most among the library defaults and the tuning candidates
   // (streaming form: 3 x 32 chunks; chain forms: up to 2048 persistent
   // workgroups; fused form: up to 128 workgroups per start group)
-  const Contract k = fused ? Contract{3, 128, 16, 8, 0} : Contract{1, 2048, 16, 8, 0};
+  const Contract k = fused ? Contract{3, 128, 16, 8, 0, 0} : Contract{1, 2048, 16, 8, 0, 0};
   const std::vector<DepthBlock> blocks =
       fused ? depth_blocks(prec, p->mwlt, 16, 0) : std::vector<DepthBlock>();
   Layout o = wloop_layout(L, p, prec, depth_fp64_starts(prec, p->mwlt, 16), lean, blocks, fused);
@@ -725,22 +730,22 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
         if constexpr (sizeof(TS) == 4) {
           if (b.kind == BLK_STORED)
             launch_contract_stored(L, msl, R, b.Ta, b.d, k.starts, k.wg, (const float2*)bptr[q].lta, S,
-                                   part, (long)pstride, c->st);
+                                   part, (long)pstride, k.mx, c->st);
           else if (b.kind == BLK_FP64)
             launch_contract_depth<float, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                  (const double2*)bptr[q].lta,
                                                  (const double2*)bptr[q].u0, S, part,
-                                                 (long)pstride, c->st);
+                                                 (long)pstride, k.mx, c->st);
           else
             launch_contract_depth<float, float>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                 (const float2*)bptr[q].lta,
                                                 (const float2*)bptr[q].u0, S, part,
-                                                (long)pstride, c->st);
+                                                (long)pstride, k.mx, c->st);
         } else {
           launch_contract_depth<double, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                 (const double2*)bptr[q].lta,
                                                 (const double2*)bptr[q].u0, S, part,
-                                                (long)pstride, c->st);
+                                                (long)pstride, k.mx, c->st);
         }
         CKL();
       }
@@ -1092,6 +1097,8 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
                    "projection) or 5 (one thread per link on packed FP32 links, mixed path)");
   if (param == WLG_TUNE_LEAN && value > 1)
     return set_err("lean layout must be 0 (default layout) or 1 (least device memory)");
+  if (param == WLG_TUNE_FUSED_TRACES && value > 2)
+    return set_err("fused traces must be 0 (default), 1 (vector units) or 2 (matrix cores)");
   if (param == WLG_TUNE_MIXED_RESTART && value > 1)
     return set_err("mixed restart must be 0 (FP64 blocks from T = 9) or 1 (FP32 blocks from T = 9 "
                    "restarted from the FP64 time transporter)");

@@ -73,17 +73,19 @@ void launch_ape_packed(const Lat& L, const float4* in, float4* out, double alpha
                        hipStream_t st);
 void launch_pack_links(long V, int nslot, const float2* src, float4* dst, hipStream_t st);
 void launch_unpack_links(long V, int nslot, const float4* src, float2* dst, hipStream_t st);
-// LDS bytes of a fused launch (stored: the form over stored FP32 transporters)
-size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored);
+// LDS bytes of a fused launch over d T (stored: the form over stored FP32
+// transporters)
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored, int d);
 template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
-                           double* partial, long pstride, hipStream_t st);
+                           double* partial, long pstride, bool mx, hipStream_t st);
 // fused launch over T = Ta .. Ta+d-1 with D_i(T) formed from the stored FP32
 // transporters LTa + (T-Ta) 9V (mixed path, FP32 level links)
+// (mx: the traces of every step on the matrix cores)
 void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                             const float2* LTa, const float2* const* S, double* partial, long pstride,
-                            hipStream_t st);
+                            bool mx, hipStream_t st);
 // W of one R (index r) from the partials of its (R,T) launches
 void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);

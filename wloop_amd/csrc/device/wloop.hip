@@ -15,6 +15,8 @@
 // Reduction is deterministic: wavefront butterfly, per-wave accumulators in
 // LDS, the waves of a workgroup in fixed order into one partial per
 // (R,T,x0,workgroup,i,j); a final kernel sums the partials in fixed order.
+#include <type_traits>
+
 #include "su3.hpp"
 #include "kernels.h"
 
@@ -407,6 +409,13 @@ __global__ __launch_bounds__(256, 2) void contract_chain1(Lat L, int R, int T, i
 // partner of lane p: 15 - p (mirror), 7 - p within each half of 8 (half
 // mirror), p ^ 2 and p ^ 1 (quad permutations)
 constexpr int kRowMirror = 0x140, kRowHalfMirror = 0x141, kQuadXor2 = 0x4E, kQuadXor1 = 0xB1;
+// rotation within a row: lane p reads lane (p - n) mod 16 (row_ror:n)
+constexpr int kRowRor4 = 0x124, kRowRor8 = 0x128, kRowRor12 = 0x12C;
+typedef float mfma_f32x4 __attribute__((ext_vector_type(4)));
+
+// sink 2-vectors per level in a ring slot of the fused form (16 pairs x 9
+// elements, padded by 16 dwords)
+constexpr int sink_stride(size_t v2_bytes) { return 144 + (v2_bytes == 8 ? 8 : 4); }
 
 // (every lane of these patterns has a source lane, so the moves need no
 // fallback value)
@@ -667,10 +676,28 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // no time link is accumulated in FP32).  The ring then holds the sinks only;
 // each wave keeps the source links S_i(x) of its start slice in registers and
 // its own L_T at x and x+Rk in a two-slot LDS buffer, loaded one step ahead.
+//
+// MX (matrix-core traces): lane = 4 p + i instead.  The msl^2 traces of a pair
+// are one 4x4 outer-product accumulation over the 18 real components,
+//     C_p[i][j] = sum_e Re D_i,e Re S_j,e + Im D_i,e Im S_j,e,
+// with lane 4p+i supplying D_i (A operand) and S_i (B operand, level j = i):
+// v_mfma_f32_4x4x1f32 (16 blocks of 4x4, one per pair) for FP32 open loops,
+// the output C_p[i][j] in accumulator register i of lane 4p+j, each trace a
+// sum of 18 products in FP32 as on the vector path.  The sum over the 16 pairs
+// is in FP64: lane bits 3 and 2 (the pairs of a row of 16 lanes) by two DPP
+// stages that hand half of the carried values to the partner, then lane bits 4
+// and 5 (the four rows) by v_mfma_f64_4x4x4f64 with a B operand of ones, whose
+// output lane 16j + 4i + c holds the sum of input lanes 16r + 4i + j over r.
+// FP64 open loops use v_mfma_f64_4x4x4f64 directly: block b of lanes 16r+4b+i
+// sums the products of the four pairs 4r+b (r = 0..3) in FP64, and two DPP
+// stages sum the blocks.  Each lane reads one sink link per step instead of
+// msl, and the trace arithmetic runs on the matrix cores beside the VALU work
+// of the other waves.
+//
 // waves per SIMD the LDS of the form allows at W = 4 (FP64 time links) or
 // W = 4 / 8 (FP32 time links), as register target
-template <int MSL, int W, class TS, class TC, bool STORED>
-__global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) void contract_depth(Lat L, int R, int Ta, int d, int nslot,
+template <int MSL, int W, class TS, class TC, bool STORED, bool MX>
+__global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 : 4)) void contract_depth(Lat L, int R, int Ta, int d, int nslot,
                                                          const v2<TC>* __restrict__ LTa,
                                                          const v2<TC>* __restrict__ U0,
                                                          LevelPtrs<TS> S,
@@ -679,21 +706,26 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
   static_assert(MSL <= 4, "fused time-slice form needs msl <= 4");
   constexpr int MM = MSL * MSL;
   constexpr int NT = 64 * W;
-  constexpr int NS = MSL * 9 * 16;             // sink 2-vectors of a slot: [j][e][p]
+  constexpr int NS = MSL * 9 * 16;             // sink 2-vectors of a slice: [j][e][p]
   constexpr int NU = 2 * 9 * 16;               // time-link 2-vectors: [x / x+Rk][e][p]
+  // sink 2-vectors per level in a ring slot: 144, padded so that the four
+  // levels of a pair (lanes 4p+i of the matrix-core form) sit in different
+  // LDS banks (level stride 48 dwords mod 64 for FP32, 16 for FP64)
+  constexpr int SJ = sink_stride(sizeof(v2<TS>));
+  constexpr int NSP = MSL * SJ;
   constexpr int PS = (NS + NT - 1) / NT, PU = STORED ? 0 : (NU + NT - 1) / NT;
   constexpr int PL = STORED ? (NU + 63) / 64 : 0;  // L_T 2-vectors per lane (own wave)
-  constexpr int SLOT = NS * (int)sizeof(v2<TS>) + (STORED ? 0 : NU * (int)sizeof(v2<TC>));  // bytes
+  constexpr int SLOT = NSP * (int)sizeof(v2<TS>) + (STORED ? 0 : NU * (int)sizeof(v2<TC>));  // bytes
   extern __shared__ double lds_d[];
-  double* acc = lds_d;                                   // [W][16][16]
-  long* xr_tab = (long*)(lds_d + W * 256);               // [16] x+Rk of the block's pairs
+  double* acc = lds_d;                                   // [W][d][16]: (i, j) at 4 i + j
+  long* xr_tab = (long*)(lds_d + W * d * 16);            // [16] x+Rk of the block's pairs
   char* ring = (char*)(xr_tab + 16);                     // [W+1] slots
-  v2<TC>* ltb = (v2<TC>*)(ring + (W + 1) * SLOT);        // STORED: [W][2][NU]
+  v2<TC>* ltb = (v2<TC>*)(ring + (W + 1) * SLOT);        // STORED: [W][NU]
   const int N0 = L.n[0];
   const long slice = L.st[0];
   const long V = L.V;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i = lane >> 4, p = lane & 15;
+  const int i = MX ? (lane & 3) : (lane >> 4), p = MX ? (lane >> 2) : (lane & 15);
   const int jw = blockIdx.x % nslot;
   const int x0g = (blockIdx.x / nslot) * W;
   const int x0 = x0g + w;
@@ -711,7 +743,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
   for (int m = 0; m < PS; m++) {
     const int q = threadIdx.x + m * NT;
     sj_[m] = q < NS ? q / 144 : -1;
-    sp_[m] = q & 15;
+    sp_[m] = q & 15;  // (ring index: q + sj_ (SJ - 144))
     sb_[m] = level_ptr(S, sj_[m] >= 0 ? sj_[m] : 0) + (long)((q >> 4) % 9) * V + sp_[m];
   }
 #pragma unroll
@@ -722,7 +754,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     uo_[m] = (long)((q >> 4) % 9) * V;
   }
 
-  for (int q = threadIdx.x; q < W * 256; q += NT) acc[q] = 0.0;
+  for (int q = threadIdx.x; q < W * d * 16; q += NT) acc[q] = 0.0;
 
   for (long bb = jw; bb < nb; bb += nslot) {
     const int k = (int)(bb / nbk);
@@ -775,21 +807,21 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
         v2<TS> v;
         v.x = sok_[m] ? ps[m].x : TS(0);
         v.y = sok_[m] ? ps[m].y : TS(0);
-        if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT] = v;
+        if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT + sj_[m] * (SJ - 144)] = v;
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
         v2<TC> v;
         v.x = uok_[m] ? pu[m].x : TC(0);
         v.y = uok_[m] ? pu[m].y : TC(0);
-        if (uh_[m] >= 0) ((v2<TC>*)(dst + NS * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = v;
+        if (uh_[m] >= 0) ((v2<TC>*)(dst + NSP * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = v;
       }
     };
     // (up to three slices in flight, as the register budget of the occupancy
     // target allows)
     constexpr int FILL = (sizeof(TC) == 8 && W < 8) ? 3 : 5;
 
-    if (threadIdx.x < 16) xr_tab[threadIdx.x] = xr;
+    if (w == 0 && i == 0) xr_tab[p] = xr;
     __syncthreads();  // xr_tab; the previous block's last reads of the ring
     offsets();
     // STORED: this wave's L_T elements q = lane + 64 m = (h*9 + e)*16 + pp at
@@ -819,28 +851,49 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
 #pragma unroll
       for (int m = 0; m < PL; m++) pl[m] = LTa[(long)n * 9 * V + lo_[m]];
     };
-    auto lt_put = [&](int b) {  // from registers into this wave's buffer b
+    // from registers into this wave's buffer (read only by this wave, at the
+    // top of the next step: LDS accesses of a wave take effect in program order)
+    auto lt_put = [&]() {
 #pragma unroll
       for (int m = 0; m < PL; m++) {
         const int q = lane + 64 * m;
         v2<TC> v;
         v.x = lok_[m] ? pl[m].x : TC(0);
         v.y = lok_[m] ? pl[m].y : TC(0);
-        if (q < NU) ltb[(w * 2 + b) * NU + q] = v;
+        if (q < NU) ltb[w * NU + q] = v;
       }
     };
     M3<TC> D;
     set_zero(D);
+    // D_i(T+1) = U_0(x+T0)^dag D_i(T) U_0(x+Rk+T0) from the ring slot's time
+    // links (every lane: D = 0 stays 0)
+    auto update = [&](const v2<TC>* ru) {
+      M3<TC> u;
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        v2<TC> c = ru[e * 16];
+        u.re[e] = c.x;
+        u.im[e] = c.y;
+      }
+      M3<TC> h = dag_mul(u, D);
+#pragma unroll
+      for (int e = 0; e < 9; e++) {
+        v2<TC> c = ru[(9 + e) * 16];
+        u.re[e] = c.x;
+        u.im[e] = c.y;
+      }
+      D = mul(h, u);
+    };
     if constexpr (STORED) {
       // sinks of the slices x0g+Ta+o in ring slots o = 0 .. W-1; L_Ta into
-      // buffer 0, L_{Ta+1} into registers
+      // the buffer, L_{Ta+1} into registers
 #pragma unroll FILL
       for (int o = 0; o < W; o++) {
         fetch(o);
         put(o);
       }
       lt_fetch(0);
-      lt_put(0);
+      lt_put();
       if (d > 1) lt_fetch(1);
       __syncthreads();
     } else {
@@ -865,12 +918,12 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
     if (live) {
       const char* slot = ring + (Ta == 1 ? (w + W) % (W + 1) : w) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
-      const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
+      const v2<TC>* ru = (const v2<TC>*)(slot + NSP * (int)sizeof(v2<TS>)) + p;
       M3<TC> a;
       M3<TS> t;
 #pragma unroll
       for (int e = 0; e < 9; e++) {
-        v2<TS> v = rs[(i * 9 + e) * 16];
+        v2<TS> v = rs[i * SJ + e * 16];
         t.re[e] = v.x;
         t.im[e] = v.y;
         v2<TC> c = ru[e * 16];
@@ -899,20 +952,25 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
 
     // the slice of ring offset n+W (read by wave W-1 at step n+1) is fetched
     // into registers during step n-1 and stored at the top of step n into the
-    // slot wave 0 read at step n-1, so a global load has a whole step to land
+    // slot wave 0 read at step n-1, so a global load has a whole step to land.
+    // The last step is peeled (no refill, no update of D), so that the body of
+    // every step is one branch-free block: the traces and the update of D can
+    // be interleaved by the compiler (the matrix-core traces then run beside
+    // the products of the update).  Lanes that are not live carry D = 0 and add
+    // zeros; D stays finite because every ring position holds a link or zero.
     if (d > 1) fetch(W);
-    for (int n = 0; n < d; n++) {
-      const bool more = n + 1 < d;
-      if (more) {
+    auto step = [&](int n, auto more_t) {
+      constexpr bool more = decltype(more_t)::value;
+      if constexpr (more) {
         put((n + W) % (W + 1));
         if (n + 2 < d) fetch(n + W + 1);
       }
       const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
-      const v2<TC>* ru = (const v2<TC>*)(slot + NS * (int)sizeof(v2<TS>)) + p;
+      const v2<TC>* ru = (const v2<TC>*)(slot + NSP * (int)sizeof(v2<TS>)) + p;
       if constexpr (STORED) {
         // D_i(T) = L_T(x)^dag S_i(x) L_T(x + R k) from this wave's buffer
-        const v2<TC>* lb = ltb + (w * 2 + (n & 1)) * NU + p;
+        const v2<TC>* lb = ltb + w * NU + p;
         M3<TC> a;
 #pragma unroll
         for (int e = 0; e < 9; e++) {
@@ -929,6 +987,59 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
         }
         D = mul(h, a);
       }
+      double* accn = acc + (w * d + n) * 16;
+      if constexpr (MX) {
+        // traces on the matrix cores; lane 4p+i holds the sink of level i
+        // (levels i >= msl read level 0: their outputs are not stored)
+        const v2<TS>* rj = rs + (i < MSL ? i : 0) * SJ;
+        double v;
+        int vij;
+        if constexpr (sizeof(TC) == 4) {
+          mfma_f32x4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            const v2<TS> sv = rj[e * 16];
+            c = __builtin_amdgcn_mfma_f32_4x4x1f32(D.re[e], (float)sv.x, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_4x4x1f32(D.im[e], (float)sv.y, c, 0, 0, 0);
+          }
+          if constexpr (more && !STORED) update(ru);
+          // lane 4p+j: c[i] = trace (i, j) of pair p.  Lane bit 3: keep the
+          // traces i = 2 b3 + {0,1}, hand the other two to lane ^ 8
+          const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+          const double t0 = c[0], t1 = c[1], t2 = c[2], t3 = c[3];
+          double k0 = b3 ? t2 : t0, k1 = b3 ? t3 : t1;
+          const double s0 = b3 ? t0 : t2, s1 = b3 ? t1 : t3;
+          k0 += row_dpp<kRowRor8>(s0);
+          k1 += row_dpp<kRowRor8>(s1);
+          // lane bit 2: keep i = 2 b3 + b2, the partner is lane + 4 (bit 2
+          // clear) or lane - 4 (bit 2 set)
+          double kk = b2 ? k1 : k0;
+          const double ss = b2 ? k0 : k1;
+          const double up = row_dpp<kRowRor12>(ss), dn = row_dpp<kRowRor4>(ss);
+          kk += b2 ? dn : up;
+          // rows: output lane 16j + 4i + c = sum over r of lane 16r + 4i + j
+          v = __builtin_amdgcn_mfma_f64_4x4x4f64(kk, 1.0, 0.0, 0, 0, 0);
+          vij = ((lane >> 2) & 3) * 4 + (lane >> 4);
+        } else {
+          double c = 0.0;
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            const v2<TS> sv = rj[e * 16];
+            c = __builtin_amdgcn_mfma_f64_4x4x4f64((double)D.re[e], (double)sv.x, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f64_4x4x4f64((double)D.im[e], (double)sv.y, c, 0, 0, 0);
+          }
+          if constexpr (more && !STORED) update(ru);
+          // lane 16i + 4b + j: trace (i, j) summed over the pairs 4r + b;
+          // sum over b (lane bits 3 and 2)
+          c += row_dpp<kRowRor8>(c);
+          c += row_dpp<kRowRor4>(c);
+          v = c;
+          vij = (lane >> 4) * 4 + (lane & 3);
+        }
+        // every lane of an entry holds the same sum: the read-modify-write of
+        // the lanes of one entry stores the same value
+        accn[vij] += v;
+      } else {
       // traces against the sinks S_j(x + (Ta+n) 0): products in the compute
       // type, each trace converted to FP64 before it is summed
       double tr[4];
@@ -939,7 +1050,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
           M3<TS> sj;
 #pragma unroll
           for (int e = 0; e < 9; e++) {
-            v2<TS> v = rs[(j * 9 + e) * 16];
+            v2<TS> v = rs[j * SJ + e * 16];
             sj.re[e] = v.x;
             sj.im[e] = v.y;
           }
@@ -958,41 +1069,26 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (W >= 8 ? 4 : 3)) voi
       v += row_dpp<kRowHalfMirror>(sh);
       v += row_dpp<kQuadXor2>(v);
       v += row_dpp<kQuadXor1>(v);
-      if (wave_live && i < MSL && (p & 3) == 0 && (p >> 2) < MSL)
-        acc[(w * 16 + n) * 16 + i * MSL + (p >> 2)] += v;
-      if constexpr (STORED) {
-        if (more) {
-          lt_put((n + 1) & 1);
-          if (n + 2 < d) lt_fetch(n + 2);
-        }
-      } else if (more) {
-        if (live) {
-          M3<TC> u;
-#pragma unroll
-          for (int e = 0; e < 9; e++) {
-            v2<TC> c = ru[e * 16];
-            u.re[e] = c.x;
-            u.im[e] = c.y;
-          }
-          M3<TC> h = dag_mul(u, D);
-#pragma unroll
-          for (int e = 0; e < 9; e++) {
-            v2<TC> c = ru[(9 + e) * 16];
-            u.re[e] = c.x;
-            u.im[e] = c.y;
-          }
-          D = mul(h, u);
-        }
+      if (wave_live && i < MSL && (p & 3) == 0 && (p >> 2) < MSL) accn[i * 4 + (p >> 2)] += v;
+      if constexpr (more && !STORED) update(ru);
       }
-      __syncthreads();
-    }
+      if constexpr (STORED && more) {
+        lt_put();
+        if (n + 2 < d) lt_fetch(n + 2);
+      }
+      if constexpr (more) __syncthreads();
+    };
+    for (int n = 0; n + 1 < d; n++) step(n, std::true_type());
+    step(d - 1, std::false_type());
+    __syncthreads();  // the ring of this block is read for the last time
   }
   __syncthreads();
   for (int q = threadIdx.x; q < W * d * MM; q += NT) {
     const int ww = q / (d * MM), n = (q / MM) % d, ij = q % MM;
     const int xo = x0g + ww;
     if (xo < N0)
-      partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] = acc[(ww * 16 + n) * 16 + ij];
+      partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] =
+          acc[(ww * d + n) * 16 + (ij / MSL) * 4 + ij % MSL];
   }
 }
 
@@ -1196,61 +1292,68 @@ void launch_contract_build(const Lat& L, int msl, int R, int T, const v2<TS>* LT
   }
 }
 
-size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored) {
+size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored, int d) {
   const size_t nu = (size_t)2 * 9 * 16 * 2 * tc_bytes;  // time links (or L_T) of a slice
-  const size_t slot = (size_t)msl * 9 * 16 * 2 * ts_bytes + (stored ? 0 : nu);
-  return (size_t)W * 256 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot +
-         (stored ? (size_t)W * 2 * nu : 0);
+  const size_t slot = (size_t)msl * sink_stride(2 * ts_bytes) * 2 * ts_bytes + (stored ? 0 : nu);
+  return (size_t)W * d * 16 * sizeof(double) + 16 * sizeof(long) + (size_t)(W + 1) * slot +
+         (stored ? (size_t)W * nu : 0);
 }
 
-template <int MSL, int W, class TS, class TC, bool STORED>
+template <int MSL, int W, class TS, class TC, bool STORED, bool MX>
 static void depth_launch(const Lat& L, int R, int Ta, int d, int nslot, const v2<TC>* LTa,
                          const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
                          hipStream_t st) {
-  const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC), STORED);
+  const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC), STORED, d);
   const dim3 grid((unsigned)(((L.n[0] + W - 1) / W) * nslot));
-  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED>), grid, dim3(64 * W), lds, st, L, R,
-                     Ta, d, nslot, LTa, U0, sp, partial, pstride);
+  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED, MX>), grid, dim3(64 * W), lds, st, L,
+                     R, Ta, d, nslot, LTa, U0, sp, partial, pstride);
 }
 
-template <int MSL, class TS, class TC, bool STORED>
+template <int MSL, class TS, class TC, bool STORED, bool MX>
 static void depth_launch_w(const Lat& L, int R, int Ta, int d, int W, int nslot, const v2<TC>* LTa,
                            const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
                            hipStream_t st) {
   if (W >= 8)
-    depth_launch<MSL, 8, TS, TC, STORED>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 8, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
   else
-    depth_launch<MSL, 4, TS, TC, STORED>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 4, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
 }
 
-template <class TS, class TC, bool STORED>
+template <class TS, class TC, bool STORED, bool MX>
 static void depth_launch_msl(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                              const v2<TC>* LTa, const v2<TC>* U0, const LevelPtrs<TS>& sp,
                              double* partial, long pstride, hipStream_t st) {
   switch (msl) {
-    case 1: depth_launch_w<1, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 2: depth_launch_w<2, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 3: depth_launch_w<3, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    default: depth_launch_w<4, TS, TC, STORED>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 1: depth_launch_w<1, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 2: depth_launch_w<2, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 3: depth_launch_w<3, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    default: depth_launch_w<4, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
   }
 }
 
 template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
-                           double* partial, long pstride, hipStream_t st) {
+                           double* partial, long pstride, bool mx, hipStream_t st) {
   LevelPtrs<TS> sp;
   for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
-  depth_launch_msl<TS, TC, false>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
+  if (mx)
+    depth_launch_msl<TS, TC, false, true>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
+  else
+    depth_launch_msl<TS, TC, false, false>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
 }
 
 void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                             const float2* LTa, const float2* const* S, double* partial, long pstride,
-                            hipStream_t st) {
+                            bool mx, hipStream_t st) {
   LevelPtrs<float> sp;
   for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
-  depth_launch_msl<float, float, true>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial, pstride,
-                                       st);
+  if (mx)
+    depth_launch_msl<float, float, true, true>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial,
+                                               pstride, st);
+  else
+    depth_launch_msl<float, float, true, false>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial,
+                                                pstride, st);
 }
 
 // kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl
@@ -1329,14 +1432,15 @@ template void launch_contract_build<double, double>(const Lat&, int, int, int, c
                                                     double2*, double*, int, int, hipStream_t);
 template void launch_contract_depth<double, double>(const Lat&, int, int, int, int, int, int,
                                                     const double2*, const double2*,
-                                                    const double2* const*, double*, long,
+                                                    const double2* const*, double*, long, bool,
                                                     hipStream_t);
 template void launch_contract_depth<float, float>(const Lat&, int, int, int, int, int, int,
                                                   const float2*, const float2*,
-                                                  const float2* const*, double*, long, hipStream_t);
+                                                  const float2* const*, double*, long, bool,
+                                                  hipStream_t);
 template void launch_contract_depth<float, double>(const Lat&, int, int, int, int, int, int,
                                                    const double2*, const double2*,
-                                                   const float2* const*, double*, long,
+                                                   const float2* const*, double*, long, bool,
                                                    hipStream_t);
 
 }  // namespace wl
