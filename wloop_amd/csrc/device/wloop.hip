@@ -724,7 +724,8 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
   const int N0 = L.n[0];
   const long slice = L.st[0];
   const long V = L.V;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, in an SGPR
   const int i = MX ? (lane & 3) : (lane >> 4), p = MX ? (lane >> 2) : (lane & 15);
   const int jw = blockIdx.x % nslot;
   const int x0g = (blockIdx.x / nslot) * W;
@@ -772,10 +773,11 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // validity of this thread's elements are fixed per pair block.  Every
     // thread loads every one of its element positions unconditionally (a
     // position with no element, or past the end of the slice, reads a valid
-    // link: element 0 of its plane), and the store into the ring zeroes what
-    // is past the end of the slice and skips positions with no element, so the
-    // loads sit in no branch and are waited for only at the store, one step
-    // later.
+    // link: element 0 of its plane) and the store into the ring skips
+    // positions with no element, so the loads sit in no branch and are waited
+    // for only at the store, one step later.  Ring positions of pairs past the
+    // end of the slice hold such links: those pairs carry D = 0, so they
+    // contribute nothing whatever (finite) link is there.
     v2<TS> ps[PS];
     v2<TC> pu[PU1];
     long so_[PS], uoff_[PU1];
@@ -804,16 +806,16 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
       char* dst = ring + slot * SLOT;
 #pragma unroll
       for (int m = 0; m < PS; m++) {
-        v2<TS> v;
-        v.x = sok_[m] ? ps[m].x : TS(0);
-        v.y = sok_[m] ? ps[m].y : TS(0);
+        v2<TS> v;  // (component-wise: a copy of the whole element kept ps in scratch)
+        v.x = ps[m].x;
+        v.y = ps[m].y;
         if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT + sj_[m] * (SJ - 144)] = v;
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
         v2<TC> v;
-        v.x = uok_[m] ? pu[m].x : TC(0);
-        v.y = uok_[m] ? pu[m].y : TC(0);
+        v.x = pu[m].x;
+        v.y = pu[m].y;
         if (uh_[m] >= 0) ((v2<TC>*)(dst + NSP * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = v;
       }
     };
@@ -826,7 +828,8 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     offsets();
     // STORED: this wave's L_T elements q = lane + 64 m = (h*9 + e)*16 + pp at
     // its start slice, loaded unconditionally (element 0 of the plane where
-    // there is none) and zeroed at the store
+    // there is none: only pairs past the end of the slice, whose D is 0, read
+    // those)
     v2<TC> pl[PL > 0 ? PL : 1];
     long lo_[PL > 0 ? PL : 1];
     bool lok_[PL > 0 ? PL : 1];
@@ -856,11 +859,10 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     auto lt_put = [&]() {
 #pragma unroll
       for (int m = 0; m < PL; m++) {
-        const int q = lane + 64 * m;
         v2<TC> v;
-        v.x = lok_[m] ? pl[m].x : TC(0);
-        v.y = lok_[m] ? pl[m].y : TC(0);
-        if (q < NU) ltb[w * NU + q] = v;
+        v.x = pl[m].x;
+        v.y = pl[m].y;
+        if (lane + 64 * m < NU) ltb[w * NU + lane + 64 * m] = v;
       }
     };
     M3<TC> D;
