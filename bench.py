@@ -419,12 +419,15 @@ def main():
         choice[name] = autotune.tune(lat, parms, prec, cache=None, log=log)
         tuned[name] = describe(choice[name])
     # the plain FP64 path keeps the reference's structure: one contraction
-    # launch per (R,T) over stored transporters (streaming or chain forms)
+    # launch per (R,T) over stored transporters (streaming or chain forms;
+    # without autotuning the two-lane chain, the fastest of them at 48^3x96)
     if not args.no_autotune:
         plain = min((t for t in choice["fp64"]["trials"] if t["kernel"] != W.KERNEL_DEPTH),
                     key=lambda t: t["ms_per_rt"])
         tuned["fp64_plain"] = describe(plain)
-        autotune.apply(lat, plain, W.FP64)
+    else:
+        plain = {"kernel": W.KERNEL_CHAIN2, "wg": 0}
+    autotune.apply(lat, plain, W.FP64)
     log(f"[bench] autotuned contraction: {tuned}")
 
     w64, t64, lat64, tim64, ls64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)

@@ -476,11 +476,13 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int td = t[mixed ? WLG_TUNE_MIXED_CONTRACT_DEPTH : WLG_TUNE_FP64_CONTRACT_DEPTH];
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
-  // defaults (measured at 48^3x96): two-lane chain form for FP64; on the
-  // mixed path form 7 (fused FP32 blocks for T <= 8, the one-lane chain at
-  // T = 9 forming S_R, the fused form over the stored transporters beyond)
-  // with 4 start slices per workgroup
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 6 : 4);
+  // defaults (measured at 48^3x96): the fused time-slice form for FP64 (one
+  // launch per R and block of 16 T, 8 start slices per workgroup: 1,064
+  // against 1,624 ms per configuration for the two-lane chain); on the mixed
+  // path form 7 (fused FP32 blocks for T <= 8, the one-lane chain at T = 9
+  // forming S_R, the fused form over the stored transporters beyond) with 4
+  // start slices per workgroup
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 6 : 3);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
   static const int wg_default[7] = {8, 512, 512, 128, 512, 512, 512};  // per form, measured at 48^3x96

@@ -123,13 +123,14 @@ class Admission:
 def expected_ms(req):
     """Predicted service time of a request on one MI355X (ms): per-site costs of
     HYP and the transporters, of an APE step and of one (R,T) contraction,
-    measured at 48^3x96 (mixed path; the FP64 path is about twice as slow),
-    plus a fixed launch overhead."""
+    measured at 48^3x96 (mixed path: 4.0 + 0.25 per APE step + 0.19 per (R,T)
+    ns per site; the FP64 path takes about 1.4 times as long), plus a fixed
+    launch overhead."""
     V = float(np.prod(req.dims))
     p = req.parms
     steps = sum(p.nss[:p.msl])
-    ns_per_site = 15.0 + 0.29 * steps + 0.2 * p.mwlr * p.mwlt
-    return 20.0 + V * ns_per_site * 1e-6 * (1.0 if req.precision == W.MIXED else 2.0)
+    ns_per_site = 4.0 + 0.25 * steps + 0.19 * p.mwlr * p.mwlt
+    return 20.0 + V * ns_per_site * 1e-6 * (1.0 if req.precision == W.MIXED else 1.4)
 
 
 class Scheduler:
