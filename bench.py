@@ -422,7 +422,9 @@ def main():
     # launch per (R,T) over stored transporters (streaming or chain forms;
     # without autotuning the two-lane chain, the fastest of them at 48^3x96)
     if not args.no_autotune:
-        plain = min((t for t in choice["fp64"]["trials"] if t["kernel"] != W.KERNEL_DEPTH),
+        # (on the FP64 path form 8 is the fused form)
+        plain = min((t for t in choice["fp64"]["trials"]
+                     if t["kernel"] not in (W.KERNEL_DEPTH, W.KERNEL_STORED9)),
                     key=lambda t: t["ms_per_rt"])
         tuned["fp64_plain"] = describe(plain)
     else:
@@ -433,7 +435,7 @@ def main():
     w64, t64, lat64, tim64, ls64 = time_path(lat, parms, W.FP64, args.steps, args.warmup)
     log(f"[bench] fp64 (plain): {t64 / args.steps * 1e3:.1f} ms/config  stages {tim64}")
     best64 = None
-    if args.no_autotune or choice["fp64"]["kernel"] == W.KERNEL_DEPTH:
+    if args.no_autotune or choice["fp64"]["kernel"] in (W.KERNEL_DEPTH, W.KERNEL_STORED9):
         # the fused time-slice form for FP64 (library default shape without autotuning)
         autotune.apply(lat, {"kernel": W.KERNEL_DEPTH, "wg": 0} if args.no_autotune else choice["fp64"],
                        W.FP64)

@@ -130,9 +130,12 @@ size_t wlg_footprint_lean(const int dims[4], const wlg_parms *p, int precision);
  * fused form for the FP32 T blocks (T <= 8) of the mixed path and the one-lane
  * chain form for the T beyond, msl <= 4, 7 as 6 with the one-lane chain form
  * at T = 9 only and the fused form for T >= 10 forming the open loops afresh at
- * every T from the stored FP32 copies of the FP64 time transporters; on the
- * FP64 path forms 6 and 7 are form 5; defaults 5 on the FP64 path and 7, with
- * 4 start slices, on the mixed path) and
+ * every T from the stored FP32 copies of the FP64 time transporters, 8 the
+ * fused form for the FP32 T blocks and the fused form over the stored
+ * transporters for every T >= 9; on the FP64 path forms 6 and 7 are form 5
+ * and form 8 is form 4; in forms 4 and 8 the first fused launch of each R
+ * also forms the next R's spatial transporters; defaults 4 on the FP64 path
+ * and 8, with 4 start slices, on the mixed path) and
  * its workgroup parameter (chunks per
  * (x0,k) for the streaming form, persistent workgroups for the chain forms,
  * workgroups per group of start slices for the fused form; form 6 uses it for

@@ -22,7 +22,7 @@ def test_contraction_variants_agree(wl, precision):
         for kernel, wgs in ((wl.KERNEL_STREAM, (1, 3, 64)), (wl.KERNEL_CHAIN, (1, 7, 512)),
                             (wl.KERNEL_CHAIN1, (1, 5, 256)), (wl.KERNEL_DEPTH, (1, 8, 40)),
                             (wl.KERNEL_CHAIN2, (1, 6, 512)), (wl.KERNEL_HYBRID, (1, 8, 40)),
-                            (wl.KERNEL_STORED, (1, 8, 40))):
+                            (wl.KERNEL_STORED, (1, 8, 40)), (wl.KERNEL_STORED9, (1, 8, 40))):
             for wg in wgs:
                 lat.tune(pk, kernel)
                 lat.tune(pk + 1, wg)
@@ -42,7 +42,7 @@ def test_contraction_variants_agree(wl, precision):
 @pytest.mark.parametrize("dims,bc,mwlt,mwlr", [((10, 5, 6, 7), 0, 5, 2), ((20, 5, 6, 7), 3, 11, 3),
                                                ((18, 4, 4, 6), 0, 10, 2), ((36, 4, 4, 6), 3, 16, 2)])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("kernel", [4, 6, 7])
+@pytest.mark.parametrize("kernel", [4, 6, 7, 8])
 @pytest.mark.parametrize("traces", [1, 2])
 def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel, traces):
     nss = [0, 2, 3]
@@ -77,7 +77,7 @@ def test_fused_time_slices(wl, dims, bc, mwlt, mwlr, precision, kernel, traces):
 # slice (ragged last pair block)
 @pytest.mark.parametrize("nss", [[0], [0, 2], [0, 1, 2], [1, 1, 2, 3]])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("kernel", [4, 7])
+@pytest.mark.parametrize("kernel", [4, 7, 8])
 def test_fused_traces_matrix_cores(wl, nss, precision, kernel):
     dims, bc, mwlt, mwlr = (26, 5, 6, 7), 3, 12, 3
     u = O.generate(dims, bc, 47, 0.3)
@@ -162,7 +162,7 @@ def test_two_lane_chain(wl, nss, bc, precision):
 
 def test_fused_rejects_bad_shapes(wl):
     with wl.Lattice((8, 4, 4, 4), 3) as lat:
-        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 8), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
+        for param, value in ((wl.TUNE_FP64_CONTRACT_KERNEL, 9), (wl.TUNE_FP64_CONTRACT_DEPTH, 17),
                              (wl.TUNE_FUSED_TRACES, 3),
                              (wl.TUNE_MIXED_CONTRACT_STARTS, 3), (wl.TUNE_FP64_CONTRACT_STARTS, 2)):
             with pytest.raises(wl.WloopError):

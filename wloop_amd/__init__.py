@@ -21,13 +21,15 @@ T_HYP, T_LEVELS, T_TIME_TR, T_SPACE_TR, T_CONTRACT, T_FINISH = range(6)
  TUNE_FP64_CONTRACT_DEPTH, TUNE_FP64_CONTRACT_STARTS, TUNE_MIXED_CONTRACT_DEPTH,
  TUNE_MIXED_CONTRACT_STARTS, TUNE_APE_FORM, TUNE_LEAN, TUNE_MIXED_RESTART, TUNE_FUSED_TRACES) = range(12)
 (KERNEL_STREAM, KERNEL_CHAIN, KERNEL_CHAIN1, KERNEL_DEPTH, KERNEL_CHAIN2, KERNEL_HYBRID,
- KERNEL_STORED) = 1, 2, 3, 4, 5, 6, 7
+ KERNEL_STORED, KERNEL_STORED9) = 1, 2, 3, 4, 5, 6, 7, 8
 KERNEL_NAMES = {KERNEL_STREAM: "streaming", KERNEL_CHAIN: "chain (msl lanes per pair)",
                 KERNEL_CHAIN1: "chain (one lane per pair)", KERNEL_DEPTH: "fused time slices",
                 KERNEL_CHAIN2: "chain (two lanes per pair)",
                 KERNEL_HYBRID: "fused time slices for T <= 8, one-lane chain beyond",
                 KERNEL_STORED: "fused time slices for T <= 8, one-lane chain at T = 9, fused over "
-                               "the stored transporters from T = 10"}
+                               "the stored transporters from T = 10",
+                KERNEL_STORED9: "fused time slices for T <= 8 forming the next R's spatial "
+                                "transporters, fused over the stored transporters from T = 9"}
 STAGES = ("hyp", "levels", "time_transporter", "space_transporter", "contract", "finish")
 L_FUSED, L_CHAIN, L_CHAIN_BUILD, L_FINISH, L_STORED = range(5)
 LAUNCH_KINDS = {L_FUSED: "fused", L_CHAIN: "chain", L_CHAIN_BUILD: "chain_build", L_FINISH: "finish",

@@ -29,7 +29,8 @@ def _cands():
     out += [{"kernel": W.KERNEL_DEPTH, "wg": wg, "depth": d, "starts": st}
             for d in (8, 16) for st in (4, 8) for wg in (32, 64, 128)]
     out += [{"kernel": kern, "wg": wg, "depth": 8, "starts": st}
-            for kern in (W.KERNEL_HYBRID, W.KERNEL_STORED) for st in (4, 8) for wg in (256, 512)]
+            for kern in (W.KERNEL_HYBRID, W.KERNEL_STORED, W.KERNEL_STORED9) for st in (4, 8)
+            for wg in (256, 512)]
     return out
 
 

@@ -465,6 +465,13 @@ static std::vector<DepthBlock> fused_blocks(const Contract& k, int prec, int mwl
   if (k.kernel == 6 && prec == WLG_MIXED)
     for (int T = kTimeFp64From + 1; T <= mwlt; T += 16)
       b.push_back({T, mwlt - T + 1 < 16 ? mwlt - T + 1 : 16, BLK_STORED});
+  if (k.kernel == 7) {  // form 8: the FP32 blocks, then every T from 9 over the stored transporters
+    if (prec != WLG_MIXED) return depth_blocks(prec, mwlt, k.depth, 0);
+    for (const DepthBlock& x : depth_blocks(prec, mwlt, k.depth, 0))
+      if (x.kind == BLK_FP32) b.push_back(x);
+    for (int T = kTimeFp64From; T <= mwlt; T += 16)
+      b.push_back({T, mwlt - T + 1 < 16 ? mwlt - T + 1 : 16, BLK_STORED});
+  }
   return b;
 }
 
@@ -477,15 +484,15 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
   // defaults (measured at 48^3x96): the fused time-slice form for FP64 (one
-  // launch per R and block of 16 T, 8 start slices per workgroup: 1,064
-  // against 1,624 ms per configuration for the two-lane chain); on the mixed
-  // path form 7 (fused FP32 blocks for T <= 8, the one-lane chain at T = 9
-  // forming S_R, the fused form over the stored transporters beyond) with 4
-  // start slices per workgroup
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 6 : 3);
+  // launch per R and block of 16 T, 8 start slices per workgroup: 1,038
+  // against 1,600 ms per configuration for the two-lane chain); on the mixed
+  // path form 8 (fused FP32 blocks for T <= 8 forming the next R's spatial
+  // transporters, the fused form over the stored transporters from T = 9:
+  // 768 against 777 ms for form 7) with 4 start slices per workgroup
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 7 : 3);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[7] = {8, 512, 512, 128, 512, 512, 512};  // per form, measured at 48^3x96
+  static const int wg_default[8] = {8, 512, 512, 128, 512, 512, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
   k.starts = ts > 0 ? ts : (mixed ? 4 : 8);
@@ -684,6 +691,10 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     if (!covered[T]) tchain.push_back(T);
   // forms 5 and 6 continue with the one-lane chain (mixed) / two-lane chain (FP64)
   const int kc = (k.kernel == 5 || k.kernel == 6) ? (sizeof(TS) == 4 ? 2 : 4) : k.kernel;
+  // forms 4 and 8 (every T fused): the first fused launch of each R forms the
+  // next R's spatial transporters for its start slices (S_2 in the launches of
+  // R = 1, from the levels), so no launch of its own forms them
+  const bool ahead = (k.kernel == 3 || k.kernel == 7) && tchain.empty();
   // with the one-lane (FP32 level links) or two-lane (FP64) chain form for the
   // per-(R,T) launches, the first of them forms the spatial transporters of its
   // R (no separate pass over F, S_{R-1} and S_R), ahead of every other launch
@@ -700,6 +711,8 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     const bool build_r = build && R > 1;
     if (R == 1) {
       for (int i = 0; i < msl; i++) S[i] = lev[i];
+    } else if (ahead) {
+      for (int i = 0; i < msl; i++) S[i] = bufs[R % 2] + i * lvl;
     } else {
       v2<TS>* out = bufs[R % 2];
       for (int i = 0; i < msl; i++) prev[i] = (R == 2) ? lev[i] : bufs[(R - 1) % 2] + i * lvl;
@@ -722,6 +735,7 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
     }
     if (record(c, &eb)) return -1;
     // fused blocks in two groups, the blocks over stored transporters last
+    v2<TS>* snext = (ahead && R < p->mwlr) ? bufs[(R + 1) % 2] : nullptr;
     for (int group = 0; group < 2; group++) {
       long nl = 0;
       for (size_t q = 0; q < blocks.size(); q++) {
@@ -729,24 +743,26 @@ static int run_transport_contract(wlg_ctx* c, const wlg_parms* p, const v2<TS>* 
         if ((b.kind == BLK_STORED) != (group == 1)) continue;
         double* part = c->partial + (size_t)(b.Ta - 1) * pstride;
         nl++;
+        v2<TS>* sn = snext;  // formed by the first launch of the R
+        snext = nullptr;
         if constexpr (sizeof(TS) == 4) {
           if (b.kind == BLK_STORED)
             launch_contract_stored(L, msl, R, b.Ta, b.d, k.starts, k.wg, (const float2*)bptr[q].lta, S,
-                                   part, (long)pstride, k.mx, c->st);
+                                   lev, sn, part, (long)pstride, k.mx, c->st);
           else if (b.kind == BLK_FP64)
             launch_contract_depth<float, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                  (const double2*)bptr[q].lta,
-                                                 (const double2*)bptr[q].u0, S, part,
+                                                 (const double2*)bptr[q].u0, S, lev, sn, part,
                                                  (long)pstride, k.mx, c->st);
           else
             launch_contract_depth<float, float>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                 (const float2*)bptr[q].lta,
-                                                (const float2*)bptr[q].u0, S, part,
+                                                (const float2*)bptr[q].u0, S, lev, sn, part,
                                                 (long)pstride, k.mx, c->st);
         } else {
           launch_contract_depth<double, double>(L, msl, R, b.Ta, b.d, k.starts, k.wg,
                                                 (const double2*)bptr[q].lta,
-                                                (const double2*)bptr[q].u0, S, part,
+                                                (const double2*)bptr[q].u0, S, lev, sn, part,
                                                 (long)pstride, k.mx, c->st);
         }
         CKL();
@@ -1081,12 +1097,13 @@ extern "C" int wlg_tune(wlg_ctx* c, int param, int value) {
                    "them before changing a tuning parameter", c->batch_refs);
   if (value < 0) return set_err("tuning value must be >= 0 (0 = default)");
   if ((param == WLG_TUNE_FP64_CONTRACT_KERNEL || param == WLG_TUNE_MIXED_CONTRACT_KERNEL) &&
-      value > 7)
+      value > 8)
     return set_err("contraction kernel must be 0 (default), 1 (streaming), 2 (chain, msl lanes "
                    "per pair), 3 (chain, one lane per pair), 4 (fused time slices), 5 (chain, "
                    "two lanes per pair), 6 (fused time slices for the FP32 T blocks of the "
-                   "mixed path, chain form beyond) or 7 (as 6, with the T blocks from 10 fused "
-                   "over the stored transporters)");
+                   "mixed path, chain form beyond), 7 (as 6, with the T blocks from 10 fused "
+                   "over the stored transporters) or 8 (fused time slices for the FP32 T blocks, "
+                   "fused over the stored transporters from T = 9)");
   if ((param == WLG_TUNE_FP64_CONTRACT_DEPTH || param == WLG_TUNE_MIXED_CONTRACT_DEPTH) &&
       value > 16)
     return set_err("time-slice depth must be 0 (default) or 1..16");

@@ -79,13 +79,14 @@ size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored
 template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
-                           double* partial, long pstride, bool mx, hipStream_t st);
+                           const v2<TS>* const* F, v2<TS>* snext, double* partial, long pstride,
+                           bool mx, hipStream_t st);
 // fused launch over T = Ta .. Ta+d-1 with D_i(T) formed from the stored FP32
 // transporters LTa + (T-Ta) 9V (mixed path, FP32 level links)
 // (mx: the traces of every step on the matrix cores)
 void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
-                            const float2* LTa, const float2* const* S, double* partial, long pstride,
-                            bool mx, hipStream_t st);
+                            const float2* LTa, const float2* const* S, const float2* const* F,
+                            float2* snext, double* partial, long pstride, bool mx, hipStream_t st);
 // W of one R (index r) from the partials of its (R,T) launches
 void launch_finish_wloop(int N0, int msl, int mwlt, int mwlr, int r, int nwg, const double* partial,
                          double norm, double* wl, hipStream_t st);

@@ -701,6 +701,8 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
                                                          const v2<TC>* __restrict__ LTa,
                                                          const v2<TC>* __restrict__ U0,
                                                          LevelPtrs<TS> S,
+                                                         LevelPtrs<TS> F,
+                                                         v2<TS>* __restrict__ snext,
                                                          double* __restrict__ partial,
                                                          long pstride) {
   static_assert(MSL <= 4, "fused time-slice form needs msl <= 4");
@@ -768,6 +770,21 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     const long xr = fwd(L, pair_live ? xs : 0, x, k + 1, R);  // spatial index of x + R k
     const long kV = (long)9 * k * V;
 
+    // the next R's spatial transporters at this wave's start slice (snext:
+    // msl 3-slot fields), S_{R+1}(x,k) = F(x,k) S_R(x+k,k) in the product and
+    // operand order of space_transporter: the start slices and pair blocks of a
+    // launch cover every (site, k) once, so the launch forms every S_{R+1} once
+    // and no separate pass over F, S_R and S_{R+1} is needed.  (Loading the
+    // operands early, to overlap the ring fill or the block's last step, made
+    // the compiler spill 30-60 registers.)
+    if (snext != nullptr && live) {
+      const long xsl = (long)x0 * slice;
+      M3<TS> f, sr;
+      load(f, level_ptr(F, i), V, k, xsl + xs);
+      load(sr, level_ptr(S, i), V, k, xsl + fwd(L, xs, x, k + 1, 1));
+      store(mul(f, sr), snext + (long)i * 27 * V, V, k, xsl + xs);
+    }
+
     // one time slice of the block (sinks S_j and the links of usrc at x and
     // x + R k) into registers / from registers into a ring slot.  Offsets and
     // validity of this thread's elements are fixed per pair block.  Every
@@ -780,13 +797,14 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // contribute nothing whatever (finite) link is there.
     v2<TS> ps[PS];
     v2<TC> pu[PU1];
-    long so_[PS], uoff_[PU1];
+    const v2<TS>* sa_[PS];  // the element's link at the pair block's first slice
+    long uoff_[PU1];
     bool sok_[PS], uok_[PU1];
     auto offsets = [&]() {
 #pragma unroll
       for (int m = 0; m < PS; m++) {
         sok_[m] = sj_[m] >= 0 && xs0 + sp_[m] < slice;
-        so_[m] = kV + (sok_[m] ? xs0 : -(long)sp_[m]);
+        sa_[m] = sb_[m] + kV + (sok_[m] ? xs0 : -(long)sp_[m]);
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
@@ -796,7 +814,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     };
     auto fetch_at = [&](long ys, const v2<TC>* __restrict__ usrc) {
 #pragma unroll
-      for (int m = 0; m < PS; m++) ps[m] = sb_[m][so_[m] + ys];
+      for (int m = 0; m < PS; m++) ps[m] = sa_[m][ys];
 #pragma unroll
       for (int m = 0; m < PU; m++) pu[m] = usrc[uoff_[m] + ys];
     };
@@ -831,7 +849,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // there is none: only pairs past the end of the slice, whose D is 0, read
     // those)
     v2<TC> pl[PL > 0 ? PL : 1];
-    long lo_[PL > 0 ? PL : 1];
+    unsigned lo_[PL > 0 ? PL : 1];  // element offsets within one L_T (< 9V < 2^32)
     bool lok_[PL > 0 ? PL : 1];
     M3<TC> Msrc;
     if constexpr (STORED) {
@@ -841,7 +859,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
         const int q = lane + 64 * m;
         const int hh = q / 144, e = (q >> 4) % 9, pp = q & 15;
         lok_[m] = q < NU && xs0 + pp < slice;
-        lo_[m] = lok_[m] ? (long)e * V + xslice + (hh ? xr_tab[pp] : xs0 + pp) : 0;
+        lo_[m] = lok_[m] ? (unsigned)((long)e * V + xslice + (hh ? xr_tab[pp] : xs0 + pp)) : 0u;
       }
       set_zero(Msrc);
       if (live) {
@@ -852,7 +870,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     }
     auto lt_fetch = [&](int n) {  // L_{Ta+n} of this wave's pairs into registers
 #pragma unroll
-      for (int m = 0; m < PL; m++) pl[m] = LTa[(long)n * 9 * V + lo_[m]];
+      for (int m = 0; m < PL; m++) pl[m] = (LTa + (long)n * 9 * V)[lo_[m]];
     };
     // from registers into this wave's buffer (read only by this wave, at the
     // top of the next step: LDS accesses of a wave take effect in program order)
@@ -988,6 +1006,9 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
           a.im[e] = c.y;
         }
         D = mul(h, a);
+        // (the traces' sink reads stay below: hoisted above the formation of
+        // D they hold 18 more registers there)
+        __builtin_amdgcn_sched_barrier(0);
       }
       double* accn = acc + (w * d + n) * 16;
       if constexpr (MX) {
@@ -1301,61 +1322,79 @@ size_t contract_depth_lds(int msl, int W, int ts_bytes, int tc_bytes, int stored
          (stored ? (size_t)W * nu : 0);
 }
 
+// the pointers of a fused launch besides its time links
+template <class TS>
+struct DepthArgs {
+  LevelPtrs<TS> sp, fp;  // S_R and the operator levels F
+  v2<TS>* snext;         // S_{R+1} formed by the launch, or null
+  double* partial;
+  long pstride;
+};
+
 template <int MSL, int W, class TS, class TC, bool STORED, bool MX>
 static void depth_launch(const Lat& L, int R, int Ta, int d, int nslot, const v2<TC>* LTa,
-                         const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
-                         hipStream_t st) {
+                         const v2<TC>* U0, const DepthArgs<TS>& a, hipStream_t st) {
   const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC), STORED, d);
   const dim3 grid((unsigned)(((L.n[0] + W - 1) / W) * nslot));
   hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED, MX>), grid, dim3(64 * W), lds, st, L,
-                     R, Ta, d, nslot, LTa, U0, sp, partial, pstride);
+                     R, Ta, d, nslot, LTa, U0, a.sp, a.fp, a.snext, a.partial, a.pstride);
 }
 
 template <int MSL, class TS, class TC, bool STORED, bool MX>
 static void depth_launch_w(const Lat& L, int R, int Ta, int d, int W, int nslot, const v2<TC>* LTa,
-                           const v2<TC>* U0, const LevelPtrs<TS>& sp, double* partial, long pstride,
-                           hipStream_t st) {
+                           const v2<TC>* U0, const DepthArgs<TS>& a, hipStream_t st) {
   if (W >= 8)
-    depth_launch<MSL, 8, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 8, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, a, st);
   else
-    depth_launch<MSL, 4, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch<MSL, 4, TS, TC, STORED, MX>(L, R, Ta, d, nslot, LTa, U0, a, st);
 }
 
 template <class TS, class TC, bool STORED, bool MX>
 static void depth_launch_msl(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
-                             const v2<TC>* LTa, const v2<TC>* U0, const LevelPtrs<TS>& sp,
-                             double* partial, long pstride, hipStream_t st) {
+                             const v2<TC>* LTa, const v2<TC>* U0, const DepthArgs<TS>& a,
+                             hipStream_t st) {
   switch (msl) {
-    case 1: depth_launch_w<1, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 2: depth_launch_w<2, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    case 3: depth_launch_w<3, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
-    default: depth_launch_w<4, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st); break;
+    case 1: depth_launch_w<1, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, a, st); break;
+    case 2: depth_launch_w<2, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, a, st); break;
+    case 3: depth_launch_w<3, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, a, st); break;
+    default: depth_launch_w<4, TS, TC, STORED, MX>(L, R, Ta, d, W, nslot, LTa, U0, a, st); break;
   }
+}
+
+template <class TS>
+static DepthArgs<TS> depth_args(int msl, const v2<TS>* const* S, const v2<TS>* const* F,
+                                v2<TS>* snext, double* partial, long pstride) {
+  DepthArgs<TS> a;
+  for (int i = 0; i < 5; i++) {
+    a.sp.p[i] = i < msl ? S[i] : nullptr;
+    a.fp.p[i] = (snext && i < msl) ? F[i] : nullptr;
+  }
+  a.snext = snext;
+  a.partial = partial;
+  a.pstride = pstride;
+  return a;
 }
 
 template <class TS, class TC>
 void launch_contract_depth(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
                            const v2<TC>* LTa, const v2<TC>* U0, const v2<TS>* const* S,
-                           double* partial, long pstride, bool mx, hipStream_t st) {
-  LevelPtrs<TS> sp;
-  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
+                           const v2<TS>* const* F, v2<TS>* snext, double* partial, long pstride,
+                           bool mx, hipStream_t st) {
+  const DepthArgs<TS> a = depth_args<TS>(msl, S, F, snext, partial, pstride);
   if (mx)
-    depth_launch_msl<TS, TC, false, true>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch_msl<TS, TC, false, true>(L, msl, R, Ta, d, W, nslot, LTa, U0, a, st);
   else
-    depth_launch_msl<TS, TC, false, false>(L, msl, R, Ta, d, W, nslot, LTa, U0, sp, partial, pstride, st);
+    depth_launch_msl<TS, TC, false, false>(L, msl, R, Ta, d, W, nslot, LTa, U0, a, st);
 }
 
 void launch_contract_stored(const Lat& L, int msl, int R, int Ta, int d, int W, int nslot,
-                            const float2* LTa, const float2* const* S, double* partial, long pstride,
-                            bool mx, hipStream_t st) {
-  LevelPtrs<float> sp;
-  for (int i = 0; i < 5; i++) sp.p[i] = i < msl ? S[i] : nullptr;
+                            const float2* LTa, const float2* const* S, const float2* const* F,
+                            float2* snext, double* partial, long pstride, bool mx, hipStream_t st) {
+  const DepthArgs<float> a = depth_args<float>(msl, S, F, snext, partial, pstride);
   if (mx)
-    depth_launch_msl<float, float, true, true>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial,
-                                               pstride, st);
+    depth_launch_msl<float, float, true, true>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, a, st);
   else
-    depth_launch_msl<float, float, true, false>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, sp, partial,
-                                                pstride, st);
+    depth_launch_msl<float, float, true, false>(L, msl, R, Ta, d, W, nslot, LTa, nullptr, a, st);
 }
 
 // kernel 0: streaming form with wg chunks per (x0,k); 1: chain form (msl
@@ -1434,15 +1473,15 @@ template void launch_contract_build<double, double>(const Lat&, int, int, int, c
                                                     double2*, double*, int, int, hipStream_t);
 template void launch_contract_depth<double, double>(const Lat&, int, int, int, int, int, int,
                                                     const double2*, const double2*,
-                                                    const double2* const*, double*, long, bool,
-                                                    hipStream_t);
+                                                    const double2* const*, const double2* const*,
+                                                    double2*, double*, long, bool, hipStream_t);
 template void launch_contract_depth<float, float>(const Lat&, int, int, int, int, int, int,
                                                   const float2*, const float2*,
-                                                  const float2* const*, double*, long, bool,
-                                                  hipStream_t);
+                                                  const float2* const*, const float2* const*,
+                                                  float2*, double*, long, bool, hipStream_t);
 template void launch_contract_depth<float, double>(const Lat&, int, int, int, int, int, int,
                                                    const double2*, const double2*,
-                                                   const float2* const*, double*, long, bool,
-                                                   hipStream_t);
+                                                   const float2* const*, const float2* const*,
+                                                   float2*, double*, long, bool, hipStream_t);
 
 }  // namespace wl
