@@ -97,9 +97,12 @@ KERNEL_LABELS = {
     ("ape_step", "fp64"): "ape_step<double, double, false>",
     ("hyp", "mixed"): "hyp_level1/2/3<float, double, float> (3 launches)",
     ("hyp", "fp64"): "hyp_level1/2/3<double, double, double> (3 launches)",
-    ("fused", "mixed"): "contract_depth<4, {W}, float, float, false> (fused time slices, FP32 blocks)",
-    ("fused", "fp64"): "contract_depth<4, {W}, double, double, false> (fused time slices)",
-    ("stored", "mixed"): "contract_depth<4, {W}, float, float, true> (fused over the stored transporters)",
+    ("fused", "mixed"): "contract_depth<4, {W}, float, float, false, true> (fused time slices, FP32 blocks; "
+                        "traces on the matrix cores)",
+    ("fused", "fp64"): "contract_depth<4, {W}, double, double, false, true> (fused time slices; traces on the "
+                       "matrix cores)",
+    ("stored", "mixed"): "contract_depth<4, {W}, float, float, true, true> (fused over the stored transporters; "
+                         "traces on the matrix cores)",
     ("chain", "mixed"): "contract_chain1<4, float, float, false> (one-lane chain)",
     ("chain_build", "mixed"): "contract_chain1<4, float, float, true> (one-lane chain forming S_R)",
     ("chain", "fp64"): "contract_chain2<4, double, double, *, false> (two-lane chain)",
@@ -107,7 +110,7 @@ KERNEL_LABELS = {
 }
 
 
-def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8):
+def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8, form=None):
     """One roofline entry per launch kind of a path: the APE step and HYP
     against the vector FLOP peak (FP32 without packed math and FP64 issue at
     the same rate), the fused time-slice launches against the vector FLOP peak,
@@ -117,7 +120,12 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8):
     from the levels stage over its step count, HYP from its stage per
     configuration); traffic = the HBM bytes per launch of the committed rocprof
     summary, where it has the kind.  starts: start slices per workgroup of
-    the fused launches (their template parameter W)."""
+    the fused launches (their template parameter W); form: the contraction
+    form the path ran (W.KERNEL_*): form 8 runs the stored blocks from T = 9
+    (form 7 from T = 10), and in forms 4 and 8 the first fused launch of each
+    R also forms the next R's spatial transporters (one product per level
+    link, counted in the fused launches' work)."""
+    import wloop_amd as W
     link = 72 if prec == "mixed" else 144
     msl = parms.msl
     nsteps = sum(parms.nss[:msl])
@@ -152,12 +160,16 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8):
     n_build = lstats["chain_build"][0] / steps
     n_fused = lstats["fused"][0] / steps
     n_stored = lstats.get("stored", (0, 0.0))[0] / steps
-    # T values per launch: the stored blocks cover T = 10 .. mwlt of every R
-    t_stored = max(parms.mwlt - 9, 0) * parms.mwlr / n_stored if n_stored else 0
+    # T values per launch: the stored blocks cover T = 10 (form 8: T = 9) ..
+    # mwlt of every R
+    first_stored = 9 if form == W.KERNEL_STORED9 else 10
+    t_stored = max(parms.mwlt - first_stored + 1, 0) * parms.mwlr / n_stored if n_stored else 0
     flop_per_t = V * 3 * (2 * msl * 198 + msl * msl * 36)
     if n_fused:
         t_per_launch = (pairs - n_chain - n_build - n_stored * t_stored) / n_fused
-        entry("fused", flop_per_t * t_per_launch, "flop", n_fused, lstats["fused"][1] / lstats["fused"][0])
+        ahead = V * 3 * msl * 198 * (parms.mwlr - 1) / n_fused if form in (W.KERNEL_DEPTH, W.KERNEL_STORED9) else 0
+        entry("fused", flop_per_t * t_per_launch + ahead, "flop", n_fused,
+              lstats["fused"][1] / lstats["fused"][0])
     if n_stored:
         entry("stored", flop_per_t * t_stored, "flop", n_stored, lstats["stored"][1] / lstats["stored"][0])
     if n_chain:
@@ -506,14 +518,18 @@ def main():
     if os.path.exists(prof_path) and args.workload == "c3":
         with open(prof_path) as f:
             profile = json.load(f)
-    def starts(name):  # start slices of the fused launches of a path (0: library default 8)
-        return (choice.get(name) or {}).get("starts") or 8
+    def starts(name):  # start slices of the fused launches of a path (library defaults: mixed 4, FP64 8)
+        return (choice.get(name) or {}).get("starts") or (4 if name == "mixed" else 8)
 
-    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile, starts("mixed")),
-                 "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, profile)}
+    def form(name):  # contraction form of a path (library defaults: mixed 8, FP64 4)
+        return (choice.get(name) or {}).get("kernel") or (W.KERNEL_STORED9 if name == "mixed" else W.KERNEL_DEPTH)
+
+    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile, starts("mixed"),
+                                           form("mixed")),
+                 "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, profile, 8, plain["kernel"])}
     if best64:
         rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps, profile,
-                                                  starts("fp64"))
+                                                  starts("fp64"), form("fp64"))
         best64["roofline"] = max(rooflines["fp64_best"], key=lambda e: e["ms_per_config"])
     primary = max(rooflines["mixed"], key=lambda e: e["ms_per_config"])
 

@@ -49,3 +49,30 @@ def test_sq_table_utilisation():
     _, d = ps.sq_table(sq, {"ape_step/mixed": 2.0})
     assert abs(d["ape_step/mixed"]["valu_issue_util"] - 4e9 / (1024 * 2.4e9 * 2e-3)) < 1e-12
     assert abs(d["ape_step/mixed"]["fma_share"] - 0.6) < 1e-12
+
+
+@pytest.mark.parametrize("form", [W.KERNEL_STORED, W.KERNEL_STORED9])
+def test_kernel_rooflines_per_form(form):
+    """algorithmic work per launch kind at 48^3x96: form 8 runs the stored
+    launches over T = 9..16 and forms S_{R+1} in the fused launches; form 7
+    runs T = 9 as a chain launch (forming S_R for R >= 2) and the stored
+    launches over T = 10..16"""
+    sys.path.insert(0, ROOT)
+    import bench
+    parms = W.make_parms([0, 16, 32, 48], 16, 16)
+    V = 96 * 48 ** 3
+    timers = {"levels": 235.0, "hyp": 31.0}
+    if form == W.KERNEL_STORED9:
+        lstats = {"fused": (16, 16 * 15.5), "stored": (16, 16 * 15.0), "chain": (0, 0.0),
+                  "chain_build": (0, 0.0)}
+    else:
+        lstats = {"fused": (16, 16 * 11.6), "stored": (16, 16 * 13.6), "chain": (1, 2.0),
+                  "chain_build": (15, 15 * 6.0)}
+    out = {e["kind"]: e for e in bench.kernel_rooflines("mixed", parms, V, timers, lstats, 1, {}, 4, form)}
+    flop_t = V * 3 * (2 * 4 * 198 + 16 * 36)
+    t_stored = 8 if form == W.KERNEL_STORED9 else 7
+    assert out["stored"]["algorithmic_flop_per_launch"] == pytest.approx(flop_t * t_stored)
+    ahead = V * 3 * 4 * 198 * 15 / 16 if form == W.KERNEL_STORED9 else 0
+    assert out["fused"]["algorithmic_flop_per_launch"] == pytest.approx(flop_t * 8 + ahead)
+    assert ("chain_build" in out) == (form == W.KERNEL_STORED)
+    assert out["ape_step"]["launches_per_config"] == 96

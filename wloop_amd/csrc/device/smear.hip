@@ -43,9 +43,7 @@ __device__ __forceinline__ void staple_add(M3<TC>& v, const Lat& L, site_t s, co
   M3<TC> t = mul_dag(convert<TC>(b), convert<TC>(c));
   mul_acc(v, convert<TC>(a), t);
   const site_t xmn = bwd(L, s, x, nu);
-  int y[4] = {x[0], x[1], x[2], x[3]};
-  y[nu] = (x[nu] == 0) ? L.n[nu] - 1 : x[nu] - 1;
-  const site_t xmnpm = fwd(L, xmn, y, mu, 1);
+  const site_t xmnpm = fwd(L, xmn, x, mu, 1);  // (x - nu has the mu coordinate of x: nu != mu)
   ld_link<PK>(b, B, L.V, lm, xmn);
   ld_link<PK>(c, A, L.V, ln, xmnpm);
   ld_link<PK>(a, A, L.V, ln, xmn);

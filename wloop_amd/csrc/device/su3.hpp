@@ -561,13 +561,17 @@ __device__ __forceinline__ void coords(const Lat& L, site_t s, int x[4]) {
 }
 
 // site reached from s (coordinates x) by 0 <= d < N_mu steps in direction mu
+// coordinate mu of x by selects (no run-time index into the array)
+__device__ __forceinline__ int coord(const int x[4], int mu) {
+  return mu == 0 ? x[0] : (mu == 1 ? x[1] : (mu == 2 ? x[2] : x[3]));
+}
 __device__ __forceinline__ site_t fwd(const Lat& L, site_t s, const int x[4], int mu, int d) {
-  int y = x[mu] + d;
+  int y = coord(x, mu) + d;
   return (y >= L.n[mu]) ? s + (site_t)((long)(d - L.n[mu]) * L.st[mu]) : s + (site_t)((long)d * L.st[mu]);
 }
 
 __device__ __forceinline__ site_t bwd(const Lat& L, site_t s, const int x[4], int mu) {
-  return (x[mu] == 0) ? s + (site_t)((long)(L.n[mu] - 1) * L.st[mu]) : s - (site_t)L.st[mu];
+  return (coord(x, mu) == 0) ? s + (site_t)((long)(L.n[mu] - 1) * L.st[mu]) : s - (site_t)L.st[mu];
 }
 
 }  // namespace wl

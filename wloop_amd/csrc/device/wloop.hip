@@ -665,9 +665,12 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // carried values to the partner in each of its first two stages, then added
 // into a per-wave LDS accumulator acc[T][i][j]; one
 // partial per (T, x0, workgroup slot, i, j) is stored at the end.  Workgroup
-// b serves start group b / nslot and pair blocks b % nslot + m nslot, so the
-// workgroups that read the same pair blocks sit on the same XCD when nslot is
-// a multiple of 8.  Deterministic.  Requires msl <= 4, d <= 16.
+// b serves start group b / nslot and the blocks of 16 sites b % nslot +
+// m nslot, each for k = 0, 1, 2 in turn (the time links or transporters at x
+// of a site block are fetched from HBM for its first direction and from L2
+// for the other two), so the workgroups that read the same pair blocks sit on
+// the same XCD when nslot is a multiple of 8.  Deterministic.  Requires
+// msl <= 4, d <= 16.
 //
 // STORED (mixed path, T >= 10): the time transporters of the block's T are the
 // stored FP32 copies of the FP64-accumulated L_T (LTa + (T-Ta) 9V), and
@@ -759,9 +762,11 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
 
   for (int q = threadIdx.x; q < W * d * 16; q += NT) acc[q] = 0.0;
 
-  for (long bb = jw; bb < nb; bb += nslot) {
-    const int k = (int)(bb / nbk);
-    const long xs0 = (bb % nbk) * 16;
+  // pair block bb = 3 (site block) + k: 3 jw, 3 jw + 1, 3 jw + 2, then the
+  // site block jw + nslot, ...
+  for (long bb = 3L * jw; bb < nb; bb += (bb % 3 == 2) ? 3L * nslot - 2 : 1) {
+    const int k = (int)(bb % 3);
+    const long xs0 = (bb / 3) * 16;
     const long xs = xs0 + p;
     const bool pair_live = xs < slice;
     const bool live = wave_live && i < MSL && pair_live;

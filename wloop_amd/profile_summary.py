@@ -22,7 +22,10 @@ every kernel of unfiltered counter runs.
 VALU issue utilisation = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x launch time x
 2.4 GHz): a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles at
 full rate (FP32 without packed math and FP64 alike), 256 CUs x 4 SIMDs, at the
-chip's peak engine clock.
+chip's peak engine clock.  Where the runs hold SQ_BUSY_CYCLES, the clock the
+chip ran at is SQ_BUSY_CYCLES / (32 shader engines x launch time), and the
+utilisations are also given at that clock: VALU as above, the matrix cores as
+SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs x cycles).
 """
 import collections
 import csv
@@ -34,6 +37,7 @@ import sys
 
 SIMDS = 1024
 CLOCK_HZ = 2.4e9
+SHADER_ENGINES = 32
 
 
 def _csv(d, suffix):
@@ -160,9 +164,11 @@ def sq_counters(dirs):
 
 def sq_table(sq, avg_ms):
     cols = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_FMA_F64",
-            "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS"]
+            "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
+            "SQ_INSTS_MFMA"]
     out = ["| key | avg ms | " + " | ".join(c.replace("SQ_INSTS_", "").replace("SQ_", "") for c in cols) +
-           " | FMA share | VALU issue util |", "|---|" + "---:|" * (len(cols) + 3)]
+           " | FMA share | VALU issue util (2.4 GHz) | clock GHz | VALU util (clock) | MFMA util (clock) |",
+           "|---|" + "---:|" * (len(cols) + 6)]
     derived = {}
     for k in sorted(sq):
         c = sq[k]
@@ -170,11 +176,20 @@ def sq_table(sq, avg_ms):
         valu = c.get("SQ_INSTS_VALU", 0.0)
         fma = c.get("SQ_INSTS_VALU_FMA_F32", 0.0) + c.get("SQ_INSTS_VALU_FMA_F64", 0.0)
         util = valu * 4.0 / (SIMDS * t * 1e-3 * CLOCK_HZ) if t else None
-        derived[k] = {"fma_share": fma / valu if valu else None, "valu_issue_util": util}
+        busy = c.get("SQ_BUSY_CYCLES")
+        cyc = busy / SHADER_ENGINES if busy else None  # cycles of the launch at the running clock
+        clock = cyc / (t * 1e-3) / 1e9 if (cyc and t) else None
+        util_c = valu * 4.0 / (SIMDS * cyc) if cyc else None
+        mfma_c = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (SIMDS * cyc) if cyc else None
+        derived[k] = {"fma_share": fma / valu if valu else None, "valu_issue_util": util,
+                      "clock_ghz": clock, "valu_issue_util_at_clock": util_c, "mfma_util_at_clock": mfma_c}
+
+        def f(v):
+            return f"{v:.2f}" if v is not None else "--"
         cells = [f"{c.get(x, 0.0):.4g}" for x in cols]
         out.append(f"| {k} | {t:.3f} | " + " | ".join(cells) +
-                   f" | {derived[k]['fma_share'] or 0:.2f} | {util if util is not None else 0:.2f} |"
-                   if t else f"| {k} | -- | " + " | ".join(cells) + " | -- | -- |")
+                   f" | {f(derived[k]['fma_share'])} | {f(util)} | {f(clock)} | {f(util_c)} | {f(mfma_c)} |"
+                   if t else f"| {k} | -- | " + " | ".join(cells) + " | -- | -- | -- | -- | -- |")
     return "\n".join(out), derived
 
 
