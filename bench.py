@@ -524,12 +524,20 @@ def main():
     def form(name):  # contraction form of a path (library defaults: mixed 8, FP64 4)
         return (choice.get(name) or {}).get("kernel") or (W.KERNEL_STORED9 if name == "mixed" else W.KERNEL_DEPTH)
 
-    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, profile, starts("mixed"),
-                                           form("mixed")),
-                 "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, profile, 8, plain["kernel"])}
+    # the committed profile measured the library's default launch shapes (its
+    # command runs without autotuning): its traffic applies to a path that ran them
+    def prof_for(name, prec):
+        ch = choice.get(name)
+        same = not ch or all(ch.get(k) == v for k, v in autotune.DEFAULT_SHAPE[prec].items())
+        return profile if same else {}
+    # (the plain FP64 path: every form that runs the two-lane chain on FP64)
+    plain_prof = profile if plain["kernel"] in (W.KERNEL_CHAIN2, W.KERNEL_HYBRID, W.KERNEL_STORED) else {}
+    rooflines = {"mixed": kernel_rooflines("mixed", parms, V, timmx, lsmx, args.steps, prof_for("mixed", W.MIXED),
+                                           starts("mixed"), form("mixed")),
+                 "fp64": kernel_rooflines("fp64", parms, V, tim64, ls64, args.steps, plain_prof, 8, plain["kernel"])}
     if best64:
-        rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps, profile,
-                                                  starts("fp64"), form("fp64"))
+        rooflines["fp64_best"] = kernel_rooflines("fp64", parms, V, tim64b, ls64b, args.steps,
+                                                  prof_for("fp64", W.FP64), starts("fp64"), form("fp64"))
         best64["roofline"] = max(rooflines["fp64_best"], key=lambda e: e["ms_per_config"])
     primary = max(rooflines["mixed"], key=lambda e: e["ms_per_config"])
 

@@ -35,6 +35,13 @@ def _cands():
 
 
 CANDIDATES = _cands()
+# the library's default launch shape per precision (contract_choice in
+# device/api.hip); a trial candidate replaces it only when it is faster by more
+# than KEEP_DEFAULT_WITHIN, about the spread of single trial runs, so that the
+# choice does not follow the noise between shapes of equal speed
+DEFAULT_SHAPE = {W.MIXED: {"kernel": W.KERNEL_STORED9, "wg": 512, "depth": 8, "starts": 8},
+                 W.FP64: {"kernel": W.KERNEL_DEPTH, "wg": 64, "depth": 16, "starts": 8}}
+KEEP_DEFAULT_WITHIN = 0.02
 DEFAULT_CACHE = os.environ.get(
     "WLOOP_AMD_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "wloop_amd", "tune.json"))
 
@@ -107,6 +114,11 @@ def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=Fals
         if log:
             log(f"[autotune] {key} {cand}: {per_rt:.3f} ms per (R,T)")
     best = min(trials, key=lambda t: t["ms_per_rt"])
+    dflt = DEFAULT_SHAPE.get(precision)
+    for t in trials:
+        if dflt and all(t.get(k) == v for k, v in dflt.items()) and \
+                t["ms_per_rt"] <= best["ms_per_rt"] * (1.0 + KEEP_DEFAULT_WITHIN):
+            best = t
     choice = dict(best, trials=trials)
     apply(lat, choice, precision)
     if cache:

@@ -483,19 +483,20 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   const int td = t[mixed ? WLG_TUNE_MIXED_CONTRACT_DEPTH : WLG_TUNE_FP64_CONTRACT_DEPTH];
   const int ts = t[mixed ? WLG_TUNE_MIXED_CONTRACT_STARTS : WLG_TUNE_FP64_CONTRACT_STARTS];
   Contract k;
-  // defaults (measured at 48^3x96): the fused time-slice form for FP64 (one
-  // launch per R and block of 16 T, 8 start slices per workgroup: 1,038
-  // against 1,600 ms per configuration for the two-lane chain); on the mixed
-  // path form 8 (fused FP32 blocks for T <= 8 forming the next R's spatial
-  // transporters, the fused form over the stored transporters from T = 9:
-  // 768 against 777 ms for form 7) with 4 start slices per workgroup
+  // defaults (the autotuner's trial winners at 48^3x96): the fused time-slice
+  // form for FP64 (one launch per R and block of 16 T, 8 start slices and 64
+  // workgroups per start group: 1,050 against 1,655 ms per configuration for
+  // the two-lane chain); on the mixed path form 8 (fused FP32 blocks for T <= 8
+  // forming the next R's spatial transporters, the fused form over the stored
+  // transporters from T = 9) with 8 start slices and 512 workgroups per start
+  // group (form 7 with the same shape within 0.5%, 4 start slices 2% slower)
   k.kernel = tk > 0 ? tk - 1 : (mixed ? 7 : 3);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
-  static const int wg_default[8] = {8, 512, 512, 128, 512, 512, 512, 512};  // per form, measured at 48^3x96
+  static const int wg_default[8] = {8, 512, 512, 64, 512, 512, 512, 512};  // per form, measured at 48^3x96
   k.wg = tw > 0 ? tw : wg_default[k.kernel];
   k.depth = td > 0 ? td : 16;
-  k.starts = ts > 0 ? ts : (mixed ? 4 : 8);
+  k.starts = ts > 0 ? ts : 8;
   k.restart = mixed && t[WLG_TUNE_MIXED_RESTART] == 1;
   // traces of the fused launches on the matrix cores unless asked otherwise
   // (48^3x96: FP64 fused launch 31.4 against 36.8 ms; FP32 the same time and
