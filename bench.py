@@ -521,8 +521,8 @@ def main():
     def starts(name):  # start slices of the fused launches of a path (library defaults: mixed 4, FP64 8)
         return (choice.get(name) or {}).get("starts") or (4 if name == "mixed" else 8)
 
-    def form(name):  # contraction form of a path (library defaults: mixed 8, FP64 4)
-        return (choice.get(name) or {}).get("kernel") or (W.KERNEL_STORED9 if name == "mixed" else W.KERNEL_DEPTH)
+    def form(name):  # contraction form of a path (library defaults: mixed 7, FP64 4)
+        return (choice.get(name) or {}).get("kernel") or (W.KERNEL_STORED if name == "mixed" else W.KERNEL_DEPTH)
 
     # the committed profile measured the library's default launch shapes (its
     # command runs without autotuning): its traffic applies to a path that ran them

@@ -135,7 +135,7 @@ size_t wlg_footprint_lean(const int dims[4], const wlg_parms *p, int precision);
  * transporters for every T >= 9; on the FP64 path forms 6 and 7 are form 5
  * and form 8 is form 4; in forms 4 and 8 the first fused launch of each R
  * also forms the next R's spatial transporters; defaults 4 on the FP64 path
- * and 8 on the mixed path, both with 8 start slices) and
+ * and 7 on the mixed path, both with 8 start slices) and
  * its workgroup parameter (chunks per
  * (x0,k) for the streaming form, persistent workgroups for the chain forms,
  * workgroups per group of start slices for the fused form; form 6 uses it for

@@ -39,7 +39,7 @@ CANDIDATES = _cands()
 # device/api.hip); a trial candidate replaces it only when it is faster by more
 # than KEEP_DEFAULT_WITHIN, about the spread of single trial runs, so that the
 # choice does not follow the noise between shapes of equal speed
-DEFAULT_SHAPE = {W.MIXED: {"kernel": W.KERNEL_STORED9, "wg": 512, "depth": 8, "starts": 8},
+DEFAULT_SHAPE = {W.MIXED: {"kernel": W.KERNEL_STORED, "wg": 512, "depth": 8, "starts": 8},
                  W.FP64: {"kernel": W.KERNEL_DEPTH, "wg": 64, "depth": 16, "starts": 8}}
 KEEP_DEFAULT_WITHIN = 0.02
 DEFAULT_CACHE = os.environ.get(

@@ -486,11 +486,12 @@ static Contract contract_choice(const wlg_ctx* c, int prec) {
   // defaults (the autotuner's trial winners at 48^3x96): the fused time-slice
   // form for FP64 (one launch per R and block of 16 T, 8 start slices and 64
   // workgroups per start group: 1,050 against 1,655 ms per configuration for
-  // the two-lane chain); on the mixed path form 8 (fused FP32 blocks for T <= 8
-  // forming the next R's spatial transporters, the fused form over the stored
-  // transporters from T = 9) with 8 start slices and 512 workgroups per start
-  // group (form 7 with the same shape within 0.5%, 4 start slices 2% slower)
-  k.kernel = tk > 0 ? tk - 1 : (mixed ? 7 : 3);
+  // the two-lane chain); on the mixed path form 7 (fused FP32 blocks for
+  // T <= 8, the one-lane chain at T = 9 forming S_R, the fused form over the
+  // stored transporters from T = 10) with 8 start slices and 512 workgroups per
+  // start group (form 8, which forms S_{R+1} inside the fused FP32 launch, is
+  // within 0.5%; 4 start slices are 2% slower)
+  k.kernel = tk > 0 ? tk - 1 : (mixed ? 6 : 3);
   if (c->tune[WLG_TUNE_LEAN] == 1) k.kernel = 3;     // lean layout: fused form for every T
   if (k.kernel >= 2 && p_msl_gt4(c)) k.kernel = 0;  // one-lane and fused forms need msl <= 4
   static const int wg_default[8] = {8, 512, 512, 64, 512, 512, 512, 512};  // per form, measured at 48^3x96
