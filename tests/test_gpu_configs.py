@@ -72,11 +72,15 @@ def fixture(name):
     return np.load(os.path.join(HERE, "golden", name))
 
 
-@pytest.mark.parametrize("precision,kernel", [(0, 0), (0, 4), (1, 0), (1, 4)])
+@pytest.mark.parametrize("precision,kernel", [(0, 0), (0, 5), (1, 0), (1, 4), (1, 8)])
 def test_precision_form_32c64_vs_oracle(wl, precision, kernel):
-    """kernel 0: library default (two-lane chain FP64; fused FP32 blocks then
-    one-lane chain, S_R formed in the chain launch, mixed); 4: fused time
-    slices for every T (FP64 blocks from T = 9 on the mixed path)"""
+    """kernel 0: library default (FP64: fused time slices for every T, the
+    traces on the matrix cores, S_{R+1} formed in the fused launch; mixed: form
+    7, fused FP32 blocks, the one-lane chain at T = 9 forming S_R, fused over
+    the stored transporters beyond); 5: the two-lane chain (the plain path); 4:
+    fused time slices for every T (FP64 blocks from T = 9 on the mixed path); 8:
+    fused FP32 blocks forming S_{R+1}, fused over the stored transporters from
+    T = 9"""
     g = fixture("c3s_w_sub.npz")
     dims = (64, 32, 32, 32)
     p = wl.make_parms([0, 16, 32, 48], 16, 16, alpha_ape=0.5, proj_iter=4, alpha_action=AA)
