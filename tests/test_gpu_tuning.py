@@ -256,3 +256,28 @@ def test_lean_layout(wl, nss, precision):
         batch = wl.Batch([b], [p], precision)
         assert np.array_equal(batch.run()[0], lean)
         batch.close()
+
+
+# forms 4 and 8 form the next R's spatial transporters inside the first fused
+# launch of each R: a single R (nothing to form), several R with T blocks
+# shallower than the T range (only the first launch of an R forms them), and
+# the lean layout (one generation read while the next is written)
+@pytest.mark.parametrize("mwlr,depth,lean", [(1, 0, 0), (4, 3, 0), (3, 0, 1)])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("kernel", [4, 8])
+def test_fused_forms_next_transporters(wl, mwlr, depth, lean, precision, kernel):
+    dims, bc, mwlt, nss = (20, 8, 8, 9), 0, 10, [0, 1, 3]
+    u = O.generate(dims, bc, 48, 0.3)
+    ref, _ = O.ms10(dims, bc, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    pk, pd = ((wl.TUNE_MIXED_CONTRACT_KERNEL, wl.TUNE_MIXED_CONTRACT_DEPTH) if precision else
+              (wl.TUNE_FP64_CONTRACT_KERNEL, wl.TUNE_FP64_CONTRACT_DEPTH))
+    with wl.Lattice(dims, bc) as lat:
+        lat.upload(u)
+        lat.tune(pk, kernel)
+        lat.tune(pd, depth)
+        if lean:
+            lat.tune(wl.TUNE_LEAN, 1)
+        print(f"[next S] kernel={kernel} prec={precision} mwlr={mwlr} depth={depth} lean={lean}", flush=True)
+        w = lat.measure(p, precision)
+    assert rel_err(w, ref) < (1e-12 if precision == 0 else 1e-6)
