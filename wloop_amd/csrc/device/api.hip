@@ -451,9 +451,11 @@ static Layout wloop_layout(const Lat& L, const wlg_parms* p, int prec, int nb64,
 
 static int p_msl_gt4(const wlg_ctx* c) { return c->cur_msl > 4; }
 
-// T blocks a contraction form runs with the fused time-slice kernel: every T
-// (form 3); for forms 5 and 6 on the mixed path the FP32 blocks T <= 8, and for
-// form 6 also the blocks from T = 10 over the stored transporters; the
+// T blocks a contraction form runs with the fused time-slice kernel (internal
+// form index = tuning value - 1): every T (index 3); for indices 5 and 6 on the
+// mixed path the FP32 blocks T <= 8, and for index 6 also the blocks from
+// T = 10 over the stored transporters; for index 7 the FP32 blocks and every T
+// from 9 over the stored transporters (on the FP64 path: every T); the
 // remaining T run one launch per (R,T) with the chain form (the first of them
 // forming S_R)
 static std::vector<DepthBlock> fused_blocks(const Contract& k, int prec, int mwlt) {

@@ -672,7 +672,8 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // the same XCD when nslot is a multiple of 8.  Deterministic.  Requires
 // msl <= 4, d <= 16.
 //
-// STORED (mixed path, T >= 10): the time transporters of the block's T are the
+// STORED (mixed path, T >= 10, or T >= 9 in form 8): the time transporters of
+// the block's T are the
 // stored FP32 copies of the FP64-accumulated L_T (LTa + (T-Ta) 9V), and
 //     D_i(T) = L_T(x)^dag S_i(x) L_T(x+Rk)
 // is formed afresh at every T from them (the arithmetic of the chain forms, so
