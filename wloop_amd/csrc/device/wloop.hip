@@ -687,7 +687,8 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 // with lane 4p+i supplying D_i (A operand) and S_i (B operand, level j = i):
 // v_mfma_f32_4x4x1f32 (16 blocks of 4x4, one per pair) for FP32 open loops,
 // the output C_p[i][j] in accumulator register i of lane 4p+j, each trace a
-// sum of 18 products in FP32 as on the vector path.  The sum over the 16 pairs
+// sum of 18 products in FP32 (two chains of 9, real and imaginary parts, added
+// at the end).  The sum over the 16 pairs
 // is in FP64: lane bits 3 and 2 (the pairs of a row of 16 lanes) by two DPP
 // stages that hand half of the carried values to the partner, then lane bits 4
 // and 5 (the four rows) by v_mfma_f64_4x4x4f64 with a B operand of ones, whose
@@ -1024,13 +1025,17 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
         double v;
         int vij;
         if constexpr (sizeof(TC) == 4) {
-          mfma_f32x4 c = {0.f, 0.f, 0.f, 0.f};
+          // two independent accumulations (real and imaginary parts) halve
+          // the chain of dependent outer products (four chains measured the
+          // same as two)
+          mfma_f32x4 c = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
           for (int e = 0; e < 9; e++) {
             const v2<TS> sv = rj[e * 16];
             c = __builtin_amdgcn_mfma_f32_4x4x1f32(D.re[e], (float)sv.x, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_4x4x1f32(D.im[e], (float)sv.y, c, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f32_4x4x1f32(D.im[e], (float)sv.y, ci, 0, 0, 0);
           }
+          c += ci;
           if constexpr (more && !STORED) update(ru);
           // lane 4p+j: c[i] = trace (i, j) of pair p.  Lane bit 3: keep the
           // traces i = 2 b3 + {0,1}, hand the other two to lane ^ 8
@@ -1050,13 +1055,14 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
           v = __builtin_amdgcn_mfma_f64_4x4x4f64(kk, 1.0, 0.0, 0, 0, 0);
           vij = ((lane >> 2) & 3) * 4 + (lane >> 4);
         } else {
-          double c = 0.0;
+          double c = 0.0, ci = 0.0;
 #pragma unroll
           for (int e = 0; e < 9; e++) {
             const v2<TS> sv = rj[e * 16];
             c = __builtin_amdgcn_mfma_f64_4x4x4f64((double)D.re[e], (double)sv.x, c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f64_4x4x4f64((double)D.im[e], (double)sv.y, c, 0, 0, 0);
+            ci = __builtin_amdgcn_mfma_f64_4x4x4f64((double)D.im[e], (double)sv.y, ci, 0, 0, 0);
           }
+          c += ci;
           if constexpr (more && !STORED) update(ru);
           // lane 16i + 4b + j: trace (i, j) summed over the pairs 4r + b;
           // sum over b (lane bits 3 and 2)
