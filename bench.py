@@ -36,8 +36,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# FP64 vector peak; FP32 without packed math issues at the same rate
-VALU_PEAK_TFLOPS = 78.6
+# MI355X vector peaks.  FP32 arithmetic in its 4-byte (VOP2) forms issues at
+# twice the FP64 rate on gfx950 without packed math (v_fmac/v_mul/v_add_f32
+# ~2 cycles per wave64 instruction, v_fma_f64 ~4.5), so FP32 work is priced
+# against the FP32 vector peak; a launch doing both is priced against the
+# time-weighted peak F / (F32/P32 + F64/P64)
+VALU_PEAK_FP64_TFLOPS = 78.6
+VALU_PEAK_FP32_TFLOPS = 157.3
+
+
+def valu_peak(f32, f64):
+    """the vector peak of f32 FP32 and f64 FP64 flop done together"""
+    return (f32 + f64) / (f32 / VALU_PEAK_FP32_TFLOPS + f64 / VALU_PEAK_FP64_TFLOPS)
 
 WORKLOADS = {
     "c3": dict(dims=(96, 48, 48, 48), bc=0, eps=0.25, seed=3, nss=[0, 16, 32, 48], R=16, T=16,
@@ -80,16 +90,20 @@ def time_path(lat, parms, precision, steps, warmup):
     return w, total, lat_ms, lat.timers(), lat.launch_stats()
 
 
-def hyp_flop_per_site(p):
-    """48 staple sums (4 products and the accumulation each) and 28
-    combinations (1-a)U + bS with the projection (55 + 525 p flop)"""
-    return 48 * (4 * 198 + 35) + 28 * (56 + 55 + 525 * p)
+def hyp_flop_per_site(p, mixed=False):
+    """(FP32, FP64) flop per site: 48 staple sums (4 products and the
+    accumulation each; FP32 on the mixed path) and 28 combinations
+    (1-a)U + bS with the projection (55 + 525 p flop; always FP64)"""
+    staples, comb = 48 * (4 * 198 + 35), 28 * (56 + 55 + 525 * p)
+    return (staples, comb) if mixed else (0, staples + comb)
 
 
-def ape_flop_per_site(p):
-    """3 spatial links: 8 staple products, 108 flop of accumulation and
-    combination, the projection"""
-    return 3 * (8 * 198 + 108 + 55 + 525 * p)
+def ape_flop_per_site(p, mixed=False):
+    """(FP32, FP64) flop per site, 3 spatial links: 8 staple products and 54
+    flop of their accumulation (FP32 on the mixed path), 54 flop of the
+    combination and the projection (always FP64)"""
+    staples, comb = 3 * (8 * 198 + 54), 3 * (54 + 55 + 525 * p)
+    return (staples, comb) if mixed else (0, staples + comb)
 
 
 KERNEL_LABELS = {
@@ -111,10 +125,11 @@ KERNEL_LABELS = {
 
 
 def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8, form=None):
-    """One roofline entry per launch kind of a path: the APE step and HYP
-    against the vector FLOP peak (FP32 without packed math and FP64 issue at
-    the same rate), the fused time-slice launches against the vector FLOP peak,
-    the per-(R,T) chain launches and the chain launch that forms S_R against
+    """One roofline entry per launch kind of a path: the APE step, HYP and the
+    fused time-slice launches against the vector FLOP peak of their FP32 and
+    FP64 work (valu_peak; on the mixed path the staple sums and the
+    contraction are FP32, the combinations and projections FP64), the
+    per-(R,T) chain launches and the chain launch that forms S_R against
     HBM.  achieved = algorithmic work per launch / average launch time (HIP
     events on the context's stream around each group of launches; the APE step
     from the levels stage over its step count, HYP from its stage per
@@ -134,12 +149,17 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8, f
     out = []
 
     def entry(kind, per_launch, unit_kind, launches_per_cfg, ms_per_launch, key=None):
+        """per_launch: bytes, or (FP32 flop, FP64 flop)"""
         if ms_per_launch <= 0:
             return
         if unit_kind == "flop":
+            f32, f64 = per_launch
+            per_launch = f32 + f64
             ach = per_launch / (ms_per_launch * 1e-3) / 1e12
-            e = {"bound": "valu", "achieved": ach, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": ach / VALU_PEAK_TFLOPS, "algorithmic_flop_per_launch": per_launch}
+            peak = valu_peak(f32, f64)
+            e = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                 "frac": ach / peak, "algorithmic_flop_per_launch": per_launch,
+                 "fp32_flop_per_launch": f32, "fp64_flop_per_launch": f64}
         else:
             ach = per_launch / (ms_per_launch * 1e-3) / 1e9
             e = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -152,10 +172,16 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8, f
             e["measured_hbm_GBps"] = tb / (ms_per_launch * 1e-3) / 1e9
         out.append(e)
 
+    mixed = prec == "mixed"
+
+    def flop(f, scale=1.0):  # (FP32, FP64) flop: the mixed path's contraction is FP32
+        return (f * scale, 0.0) if mixed else (0.0, f * scale)
+
     if nsteps:
-        entry("ape_step", V * ape_flop_per_site(parms.proj_iter), "flop", nsteps,
-              timers["levels"] / steps / nsteps)
-    entry("hyp", V * hyp_flop_per_site(parms.proj_iter), "flop", 1, timers["hyp"] / steps)
+        a32, a64 = ape_flop_per_site(parms.proj_iter, mixed)
+        entry("ape_step", (V * a32, V * a64), "flop", nsteps, timers["levels"] / steps / nsteps)
+    h32, h64 = hyp_flop_per_site(parms.proj_iter, mixed)
+    entry("hyp", (V * h32, V * h64), "flop", 1, timers["hyp"] / steps)
     n_chain = lstats["chain"][0] / steps
     n_build = lstats["chain_build"][0] / steps
     n_fused = lstats["fused"][0] / steps
@@ -168,10 +194,10 @@ def kernel_rooflines(prec, parms, V, timers, lstats, steps, profile, starts=8, f
     if n_fused:
         t_per_launch = (pairs - n_chain - n_build - n_stored * t_stored) / n_fused
         ahead = V * 3 * msl * 198 * (parms.mwlr - 1) / n_fused if form in (W.KERNEL_DEPTH, W.KERNEL_STORED9) else 0
-        entry("fused", flop_per_t * t_per_launch + ahead, "flop", n_fused,
+        entry("fused", flop(flop_per_t * t_per_launch + ahead), "flop", n_fused,
               lstats["fused"][1] / lstats["fused"][0])
     if n_stored:
-        entry("stored", flop_per_t * t_stored, "flop", n_stored, lstats["stored"][1] / lstats["stored"][0])
+        entry("stored", flop(flop_per_t * t_stored), "flop", n_stored, lstats["stored"][1] / lstats["stored"][0])
     if n_chain:
         entry("chain", V * (1 + 3 * msl) * link, "bytes", n_chain, lstats["chain"][1] / lstats["chain"][0])
     if n_build:
@@ -293,7 +319,7 @@ def aux_kernels(lat, steps):
         d = {"ms_per_step": ms, "precision": "fp64"}
         if flop:
             tf = flop * V / (ms * 1e-3) / 1e12
-            d.update({"flop_per_site": flop, "achieved_tflops": tf, "frac_of_fp64_vector_peak": tf / VALU_PEAK_TFLOPS})
+            d.update({"flop_per_site": flop, "achieved_tflops": tf, "frac_of_fp64_vector_peak": tf / VALU_PEAK_FP64_TFLOPS})
         out[name] = d
         log(f"[bench] {name}: {d}")
     return out

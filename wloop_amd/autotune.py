@@ -35,6 +35,15 @@ def _cands():
 
 
 CANDIDATES = _cands()
+
+
+def path_candidates(precision):
+    """the candidates that are distinct launch shapes on a path: on the FP64
+    path forms 6 and 7 run the two-lane chain (their FP32 blocks exist on the
+    mixed path only), so they repeat candidates of form 5 there"""
+    if precision == W.MIXED:
+        return CANDIDATES
+    return [c for c in CANDIDATES if c["kernel"] not in (W.KERNEL_HYBRID, W.KERNEL_STORED)]
 # the library's default launch shape per precision (contract_choice in
 # device/api.hip); a trial candidate replaces it only when it is faster by more
 # than KEEP_DEFAULT_WITHIN, about the spread of single trial runs, so that the
@@ -95,7 +104,7 @@ def tune(lat, parms, precision, cache=DEFAULT_CACHE, candidates=None, force=Fals
     # levels one APE step apart, two values of R
     trial = W.make_parms([0] + [1] * (parms.msl - 1), parms.mwlt, min(parms.mwlr, 2),
                          proj_iter=parms.proj_iter)
-    cands = candidates or CANDIDATES
+    cands = candidates or path_candidates(precision)
     trials = []
     for cand in cands:
         apply(lat, cand, precision)

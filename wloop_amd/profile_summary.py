@@ -13,19 +13,23 @@ against the FP64 -> FP32 copy kernel ``d2f`` (one complex number per thread:
 Kernels are grouped by launch kind and precision ("ape_step/mixed",
 "fused/mixed", "chain/fp64", "chain_build/mixed", ...; see kernel_key).  Prints
 markdown tables (kernel time; HBM traffic per launch; SQ counters per launch
-with the VALU issue utilisation) and writes OUT_JSON with, per key, the
+with the SIMD busy estimate) and writes OUT_JSON with, per key, the
 calibrated HBM bytes per launch, the average launch time and the SQ counters
 per launch -- the figures bench.py reads for the ``traffic`` field of its
 roofline entries.  With ``--all`` it prints the calibrated traffic and rate of
 every kernel of unfiltered counter runs.
 
-VALU issue utilisation = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x launch time x
-2.4 GHz): a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles at
-full rate (FP32 without packed math and FP64 alike), 256 CUs x 4 SIMDs, at the
-chip's peak engine clock.  Where the runs hold SQ_BUSY_CYCLES, the clock the
-chip ran at is SQ_BUSY_CYCLES / (32 shader engines x launch time), and the
-utilisations are also given at that clock: VALU as above, the matrix cores as
-SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs x cycles).
+The clock the chip ran at is SQ_BUSY_CYCLES / (32 shader engines x launch
+time).  SQ_INSTS_VALU counts the matrix-core instructions too.  SIMD busy
+estimate = (VALU cycles + SQ_VALU_MFMA_BUSY_CYCLES) / (1024 SIMDs x cycles of
+the launch), with the VALU cycles of a wave64 instruction as measured on gfx950
+by wloop_amd/issue_rates (16 waves on one CU, independent chains): 4.9 for FP64
+arithmetic (SQ_INSTS_VALU_FMA/MUL/ADD_F64) and conversions (SQ_INSTS_VALU_CVT),
+VALU_OTHER_CYCLES = 2.5 for the rest (FP32 and integer: 2.25-2.8 in the 4-byte
+encodings, with and without VGPR bank conflicts, 2.95-3.1 in the 8-byte ones;
+DPP moves 4.9 are counted at 2.5).  Matrix-core and VALU instructions of one
+SIMD do not overlap on this chip (issue_rates: 8 v_mfma_f32_4x4x1f32 and 16
+v_fmac_f32 take longer than the two apart), so the two are added.
 """
 import collections
 import csv
@@ -36,8 +40,9 @@ import re
 import sys
 
 SIMDS = 1024
-CLOCK_HZ = 2.4e9
 SHADER_ENGINES = 32
+VALU_F64_CYCLES = 4.9    # FP64 arithmetic and conversions, per wave64 instruction
+VALU_OTHER_CYCLES = 2.5  # FP32 and integer
 
 
 def _csv(d, suffix):
@@ -162,33 +167,43 @@ def sq_counters(dirs):
     return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
 
 
+def valu_cycles(c):
+    """SIMD cycles of a launch's VALU instructions (matrix-core ones excluded)
+    under the measured issue costs: FP64 arithmetic and conversions at
+    VALU_F64_CYCLES, the rest at VALU_OTHER_CYCLES"""
+    f64 = sum(c.get(x, 0.0) for x in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64",
+                                       "SQ_INSTS_VALU_CVT"))
+    rest = max(c.get("SQ_INSTS_VALU", 0.0) - c.get("SQ_INSTS_MFMA", 0.0) - f64, 0.0)
+    return VALU_F64_CYCLES * f64 + VALU_OTHER_CYCLES * rest
+
+
 def sq_table(sq, avg_ms):
     cols = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_FMA_F64",
-            "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS",
-            "SQ_INSTS_MFMA"]
+            "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_CVT",
+            "SQ_INSTS_VMEM_RD", "SQ_INSTS_LDS", "SQ_INSTS_MFMA"]
     out = ["| key | avg ms | " + " | ".join(c.replace("SQ_INSTS_", "").replace("SQ_", "") for c in cols) +
-           " | FMA share | VALU issue util (2.4 GHz) | clock GHz | VALU util (clock) | MFMA util (clock) |",
+           " | FMA share | clock GHz | VALU busy (model) | MFMA busy | SIMD busy (model) |",
            "|---|" + "---:|" * (len(cols) + 6)]
     derived = {}
     for k in sorted(sq):
         c = sq[k]
         t = avg_ms.get(k)
-        valu = c.get("SQ_INSTS_VALU", 0.0)
+        valu = c.get("SQ_INSTS_VALU", 0.0) - c.get("SQ_INSTS_MFMA", 0.0)
         fma = c.get("SQ_INSTS_VALU_FMA_F32", 0.0) + c.get("SQ_INSTS_VALU_FMA_F64", 0.0)
-        util = valu * 4.0 / (SIMDS * t * 1e-3 * CLOCK_HZ) if t else None
         busy = c.get("SQ_BUSY_CYCLES")
         cyc = busy / SHADER_ENGINES if busy else None  # cycles of the launch at the running clock
         clock = cyc / (t * 1e-3) / 1e9 if (cyc and t) else None
-        util_c = valu * 4.0 / (SIMDS * cyc) if cyc else None
-        mfma_c = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (SIMDS * cyc) if cyc else None
-        derived[k] = {"fma_share": fma / valu if valu else None, "valu_issue_util": util,
-                      "clock_ghz": clock, "valu_issue_util_at_clock": util_c, "mfma_util_at_clock": mfma_c}
+        vb = valu_cycles(c) / (SIMDS * cyc) if cyc else None
+        mb = c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / (SIMDS * cyc) if cyc else None
+        derived[k] = {"fma_share": fma / valu if valu else None, "clock_ghz": clock, "valu_busy": vb,
+                      "mfma_busy": mb, "simd_busy": (vb + mb) if vb is not None else None}
 
         def f(v):
             return f"{v:.2f}" if v is not None else "--"
         cells = [f"{c.get(x, 0.0):.4g}" for x in cols]
+        d = derived[k]
         out.append(f"| {k} | {t:.3f} | " + " | ".join(cells) +
-                   f" | {f(derived[k]['fma_share'])} | {f(util)} | {f(clock)} | {f(util_c)} | {f(mfma_c)} |"
+                   f" | {f(d['fma_share'])} | {f(clock)} | {f(vb)} | {f(mb)} | {f(d['simd_busy'])} |"
                    if t else f"| {k} | -- | " + " | ".join(cells) + " | -- | -- | -- | -- | -- |")
     return "\n".join(out), derived
 
