@@ -802,8 +802,11 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // for only at the store, one step later.  Ring positions of pairs past the
     // end of the slice hold such links: those pairs carry D = 0, so they
     // contribute nothing whatever (finite) link is there.
-    v2<TS> ps[PS];
-    v2<TC> pu[PU1];
+    struct Slice {  // this thread's elements of one slice, in registers
+      v2<TS> s[PS];
+      v2<TC> u[PU1];
+    };
+    Slice nxt;  // the slice of the next refill, fetched one step ahead
     const v2<TS>* sa_[PS];  // the element's link at the pair block's first slice
     long uoff_[PU1];
     bool sok_[PS], uok_[PU1];
@@ -819,34 +822,54 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
         uoff_[m] = uo_[m] + (uok_[m] ? (uh_[m] > 0 ? xr_tab[up_[m]] : xs0 + up_[m]) : 0);
       }
     };
-    auto fetch_at = [&](long ys, const v2<TC>* __restrict__ usrc) {
+    auto fetch_at = [&](Slice& f, long ys, const v2<TC>* __restrict__ usrc) {
 #pragma unroll
-      for (int m = 0; m < PS; m++) ps[m] = sa_[m][ys];
+      for (int m = 0; m < PS; m++) f.s[m] = sa_[m][ys];
 #pragma unroll
-      for (int m = 0; m < PU; m++) pu[m] = usrc[uoff_[m] + ys];
+      for (int m = 0; m < PU; m++) f.u[m] = usrc[uoff_[m] + ys];
     };
     // ring offset o: time slice x0g+Ta+o
-    auto fetch = [&](int o) { fetch_at((long)((x0g + Ta + o) % N0) * slice, U0); };
-    auto put = [&](int slot) {
+    auto fetch = [&](Slice& f, int o) { fetch_at(f, (long)((x0g + Ta + o) % N0) * slice, U0); };
+    auto put = [&](int slot, const Slice& f) {
       char* dst = ring + slot * SLOT;
 #pragma unroll
       for (int m = 0; m < PS; m++) {
-        v2<TS> v;  // (component-wise: a copy of the whole element kept ps in scratch)
-        v.x = ps[m].x;
-        v.y = ps[m].y;
+        v2<TS> v;  // (component-wise: a copy of the whole element kept the slice in scratch)
+        v.x = f.s[m].x;
+        v.y = f.s[m].y;
         if (sj_[m] >= 0) ((v2<TS>*)dst)[threadIdx.x + m * NT + sj_[m] * (SJ - 144)] = v;
       }
 #pragma unroll
       for (int m = 0; m < PU; m++) {
         v2<TC> v;
-        v.x = pu[m].x;
-        v.y = pu[m].y;
+        v.x = f.u[m].x;
+        v.y = f.u[m].y;
         if (uh_[m] >= 0) ((v2<TC>*)(dst + NSP * (int)sizeof(v2<TS>)))[threadIdx.x + m * NT] = v;
       }
     };
-    // (up to three slices in flight, as the register budget of the occupancy
-    // target allows)
-    constexpr int FILL = (sizeof(TC) == 8 && W < 8) ? 3 : 5;
+    // the fills of a pair block's first slices: slice j < cnt (time slice
+    // ys_of(j), time links from usrc) into ring slot slot_of(j).  The loads of
+    // FILL slices are all issued before the first of their stores: a slice's
+    // stores wait for its loads (and, behind the conditional stores, for every
+    // load before them), so filled one slice after another the slices' HBM
+    // round trips would follow one another.  FILL: as many slices as the
+    // register budget of the occupancy target leaves room for (FP32, W = 8:
+    // 5 slices of 6 registers)
+    constexpr int SLICE_REGS = (PS * (int)sizeof(v2<TS>) + PU * (int)sizeof(v2<TC>)) / 4;
+    constexpr int FILL_REGS = sizeof(TC) == 8 ? 48 : (W >= 8 ? 30 : 20);
+    constexpr int FILL = FILL_REGS / SLICE_REGS > 1 ? FILL_REGS / SLICE_REGS : 1;
+    auto fill = [&](int cnt, auto ys_of, auto slot_of, const v2<TC>* __restrict__ usrc) {
+      for (int j0 = 0; j0 < cnt; j0 += FILL) {
+        Slice f[FILL];
+#pragma unroll
+        for (int j = 0; j < FILL; j++)
+          if (j0 + j < cnt) fetch_at(f[j], ys_of(j0 + j), usrc);
+#pragma unroll
+        for (int j = 0; j < FILL; j++)
+          if (j0 + j < cnt) put(slot_of(j0 + j), f[j]);
+      }
+    };
+    auto slice_at = [&](int o) { return (long)((x0g + Ta + o) % N0) * slice; };  // ring offset o
 
     if (w == 0 && i == 0) xr_tab[p] = xr;
     __syncthreads();  // xr_tab; the previous block's last reads of the ring
@@ -914,11 +937,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     if constexpr (STORED) {
       // sinks of the slices x0g+Ta+o in ring slots o = 0 .. W-1; L_Ta into
       // the buffer, L_{Ta+1} into registers
-#pragma unroll FILL
-      for (int o = 0; o < W; o++) {
-        fetch(o);
-        put(o);
-      }
+      fill(W, slice_at, [](int j) { return j; }, U0);
       lt_fetch(0);
       lt_put();
       if (d > 1) lt_fetch(1);
@@ -928,17 +947,9 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
     // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
     if (Ta == 1) {
-#pragma unroll FILL
-      for (int o = -1; o < W; o++) {
-        fetch(o);
-        put((o + W + 1) % (W + 1));
-      }
+      fill(W + 1, [&](int j) { return slice_at(j - 1); }, [](int j) { return (j + W) % (W + 1); }, U0);
     } else {
-#pragma unroll FILL
-      for (int o = 0; o < W; o++) {
-        fetch_at((long)((x0g + o) % N0) * slice, LTa);
-        put(o);
-      }
+      fill(W, [&](int j) { return (long)((x0g + j) % N0) * slice; }, [](int j) { return j; }, LTa);
     }
     __syncthreads();
     // D_i(Ta) = L_Ta(x)^dag S_i(x) L_Ta(x + R k)
@@ -968,11 +979,7 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     }
     __syncthreads();  // ring slots of the start slices are refilled next
     if (Ta != 1) {
-#pragma unroll FILL
-      for (int o = 0; o < W; o++) {
-        fetch(o);
-        put(o);
-      }
+      fill(W, slice_at, [](int j) { return j; }, U0);
       __syncthreads();
     }
     }  // not STORED
@@ -985,12 +992,12 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // be interleaved by the compiler (the matrix-core traces then run beside
     // the products of the update).  Lanes that are not live carry D = 0 and add
     // zeros; D stays finite because every ring position holds a link or zero.
-    if (d > 1) fetch(W);
+    if (d > 1) fetch(nxt, W);
     auto step = [&](int n, auto more_t) {
       constexpr bool more = decltype(more_t)::value;
       if constexpr (more) {
-        put((n + W) % (W + 1));
-        if (n + 2 < d) fetch(n + W + 1);
+        put((n + W) % (W + 1), nxt);
+        if (n + 2 < d) fetch(nxt, n + W + 1);
       }
       const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
