@@ -29,6 +29,8 @@ def test_bench_refuses_more_than_one_gpu():
     ("void wl::contract_chain2<4, double, double, true, true>(wl::Lat, int)", "chain_build/fp64"),
     ("void wl::contract_chain<4, 4, 4, float, float>(wl::Lat, int)", "chain/mixed"),
     ("void wl::contract_depth<4, 8, float, float, false>(wl::Lat, int)", "fused/mixed"),
+    ("void wl::contract_depth<4, 8, float, float, false, true, true>(wl::Lat, int)", "fused/mixed"),
+    ("void wl::contract_depth<4, 8, float, float, true, true, false>(wl::Lat, int)", "stored/mixed"),
     ("void wl::contract_depth<4, 8, float, double, false>(wl::Lat, int)", "fused64/mixed"),
     ("void wl::contract_depth<4, 8, double, double, false>(wl::Lat, int)", "fused/fp64"),
     ("void wl::contract_depth<4, 4, float, float, true>(wl::Lat, int)", "stored/mixed"),

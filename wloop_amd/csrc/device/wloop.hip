@@ -641,6 +641,18 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 }
 
 // ---- contraction, fused time-slice form ----------------------------------
+// Whether a fused launch sweeps the start groups (one workgroup per slot
+// taking every start group of its pair blocks in turn, the ring sliding on):
+// for T from 1 with as many T as start slices per workgroup, the ring at the
+// end of a start group holds all but one of the slices the next start group
+// starts with, so the fill of W + 1 slices at each pair block becomes one
+// slice, and each slice of a pair block is fetched from HBM once.
+// (Only the FP32 form is built with the sweep; other launches keep one start
+// group per workgroup.)
+__host__ __device__ constexpr bool depth_sweep(bool stored, bool fp32, int Ta, int d, int W) {
+  return !stored && fp32 && Ta == 1 && d == W;
+}
+
 // One launch covers T = Ta .. Ta+d-1 of one R.  A lane owns (pair p of a block
 // of 16 (site,k) pairs, level i) for one start slice x0 and carries
 //     D_i(T) = L_T(x)^dag S_i(x) L_T(x+Rk)
@@ -701,7 +713,7 @@ __global__ __launch_bounds__(256, 2) void contract_chain2(Lat L, int R, int T, i
 //
 // waves per SIMD the LDS of the form allows at W = 4 (FP64 time links) or
 // W = 4 / 8 (FP32 time links), as register target
-template <int MSL, int W, class TS, class TC, bool STORED, bool MX>
+template <int MSL, int W, class TS, class TC, bool STORED, bool MX, bool SWEEP>
 __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 : 4)) void contract_depth(Lat L, int R, int Ta, int d, int nslot,
                                                          const v2<TC>* __restrict__ LTa,
                                                          const v2<TC>* __restrict__ U0,
@@ -734,10 +746,16 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform, in an SGPR
   const int i = MX ? (lane & 3) : (lane >> 4), p = MX ? (lane >> 2) : (lane & 15);
-  const int jw = blockIdx.x % nslot;
-  const int x0g = (blockIdx.x / nslot) * W;
-  const int x0 = x0g + w;
-  const bool wave_live = x0 < N0;
+  // SWEEP (depth_sweep): the workgroup takes every start group of its pair
+  // blocks in turn, with nslot workgroups in the grid; otherwise it takes one
+  // start group, with start groups x nslot workgroups
+  static_assert(!(SWEEP && STORED), "no sweep over the stored transporters");
+  constexpr bool sweep = SWEEP;
+  const int G = sweep ? (N0 + W - 1) / W : 1;
+  const int jw = sweep ? (int)blockIdx.x : (int)(blockIdx.x % nslot);
+  const int x0g0 = sweep ? 0 : (int)(blockIdx.x / nslot) * W;
+  int x0g = x0g0, x0 = x0g + w;  // the start group and this wave's start slice
+  bool wave_live = x0 < N0;
   const long nbk = (slice + 15) / 16;
   const long nb = 3 * nbk;
 
@@ -771,12 +789,20 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     const long xs0 = (bb / 3) * 16;
     const long xs = xs0 + p;
     const bool pair_live = xs < slice;
-    const bool live = wave_live && i < MSL && pair_live;
+    bool live = false;  // (per start group)
     int x[4];
     coords(L, pair_live ? xs : 0, x);
     const long xr = fwd(L, pair_live ? xs : 0, x, k + 1, R);  // spatial index of x + R k
     const long kV = (long)9 * k * V;
-
+    // the start group g (x0g0 + g W) and this wave's start slice
+    int rot = 0, g = 0;
+    auto set_group = [&](int gg) {
+      x0g = x0g0 + gg * W;
+      x0 = x0g + w;
+      wave_live = x0 < N0;
+      live = wave_live && i < MSL && pair_live;
+      rot = (W + 1 - gg % (W + 1)) % (W + 1);
+    };
     // the next R's spatial transporters at this wave's start slice (snext:
     // msl 3-slot fields), S_{R+1}(x,k) = F(x,k) S_R(x+k,k) in the product and
     // operand order of space_transporter: the start slices and pair blocks of a
@@ -784,13 +810,17 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     // and no separate pass over F, S_R and S_{R+1} is needed.  (Loading the
     // operands early, to overlap the ring fill or the block's last step, made
     // the compiler spill 30-60 registers.)
-    if (snext != nullptr && live) {
-      const long xsl = (long)x0 * slice;
-      M3<TS> f, sr;
-      load(f, level_ptr(F, i), V, k, xsl + xs);
-      load(sr, level_ptr(S, i), V, k, xsl + fwd(L, xs, x, k + 1, 1));
-      store(mul(f, sr), snext + (long)i * 27 * V, V, k, xsl + xs);
-    }
+    auto form_next = [&]() {
+      if (snext != nullptr && live) {
+        const long xsl = (long)x0 * slice;
+        M3<TS> f, sr;
+        load(f, level_ptr(F, i), V, k, xsl + xs);
+        load(sr, level_ptr(S, i), V, k, xsl + fwd(L, xs, x, k + 1, 1));
+        store(mul(f, sr), snext + (long)i * 27 * V, V, k, xsl + xs);
+      }
+    };
+    set_group(0);
+    form_next();  // (before the fill of the first start group, whose loads it then overlaps)
 
     // one time slice of the block (sinks S_j and the links of usrc at x and
     // x + R k) into registers / from registers into a ring slot.  Offsets and
@@ -882,22 +912,6 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
     unsigned lo_[PL > 0 ? PL : 1];  // element offsets within one L_T (< 9V < 2^32)
     bool lok_[PL > 0 ? PL : 1];
     M3<TC> Msrc;
-    if constexpr (STORED) {
-      const long xslice = (long)(wave_live ? x0 : 0) * slice;
-#pragma unroll
-      for (int m = 0; m < PL; m++) {
-        const int q = lane + 64 * m;
-        const int hh = q / 144, e = (q >> 4) % 9, pp = q & 15;
-        lok_[m] = q < NU && xs0 + pp < slice;
-        lo_[m] = lok_[m] ? (unsigned)((long)e * V + xslice + (hh ? xr_tab[pp] : xs0 + pp)) : 0u;
-      }
-      set_zero(Msrc);
-      if (live) {
-        M3<TS> t;
-        load(t, level_ptr(S, i), V, k, xslice + xs);
-        Msrc = convert<TC>(t);
-      }
-    }
     auto lt_fetch = [&](int n) {  // L_{Ta+n} of this wave's pairs into registers
 #pragma unroll
       for (int m = 0; m < PL; m++) pl[m] = (LTa + (long)n * 9 * V)[lo_[m]];
@@ -914,7 +928,6 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
       }
     };
     M3<TC> D;
-    set_zero(D);
     // D_i(T+1) = U_0(x+T0)^dag D_i(T) U_0(x+Rk+T0) from the ring slot's time
     // links (every lane: D = 0 stays 0)
     auto update = [&](const v2<TC>* ru) {
@@ -934,72 +947,23 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
       }
       D = mul(h, u);
     };
-    if constexpr (STORED) {
-      // sinks of the slices x0g+Ta+o in ring slots o = 0 .. W-1; L_Ta into
-      // the buffer, L_{Ta+1} into registers
-      fill(W, slice_at, [](int j) { return j; }, U0);
-      lt_fetch(0);
-      lt_put();
-      if (d > 1) lt_fetch(1);
-      __syncthreads();
-    } else {
-    // the start slices x0g+w with S_i(x) and L_Ta at x and x + R k: for Ta = 1
-    // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
-    // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
-    if (Ta == 1) {
-      fill(W + 1, [&](int j) { return slice_at(j - 1); }, [](int j) { return (j + W) % (W + 1); }, U0);
-    } else {
-      fill(W, [&](int j) { return (long)((x0g + j) % N0) * slice; }, [](int j) { return j; }, LTa);
-    }
-    __syncthreads();
-    // D_i(Ta) = L_Ta(x)^dag S_i(x) L_Ta(x + R k)
-    if (live) {
-      const char* slot = ring + (Ta == 1 ? (w + W) % (W + 1) : w) * SLOT;
-      const v2<TS>* rs = (const v2<TS>*)slot + p;
-      const v2<TC>* ru = (const v2<TC>*)(slot + NSP * (int)sizeof(v2<TS>)) + p;
-      M3<TC> a;
-      M3<TS> t;
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        v2<TS> v = rs[i * SJ + e * 16];
-        t.re[e] = v.x;
-        t.im[e] = v.y;
-        v2<TC> c = ru[e * 16];
-        a.re[e] = c.x;
-        a.im[e] = c.y;
-      }
-      const M3<TC> h = dag_mul(a, convert<TC>(t));
-#pragma unroll
-      for (int e = 0; e < 9; e++) {
-        v2<TC> c = ru[(9 + e) * 16];
-        a.re[e] = c.x;
-        a.im[e] = c.y;
-      }
-      D = mul(h, a);
-    }
-    __syncthreads();  // ring slots of the start slices are refilled next
-    if (Ta != 1) {
-      fill(W, slice_at, [](int j) { return j; }, U0);
-      __syncthreads();
-    }
-    }  // not STORED
-
-    // the slice of ring offset n+W (read by wave W-1 at step n+1) is fetched
-    // into registers during step n-1 and stored at the top of step n into the
-    // slot wave 0 read at step n-1, so a global load has a whole step to land.
-    // The last step is peeled (no refill, no update of D), so that the body of
-    // every step is one branch-free block: the traces and the update of D can
-    // be interleaved by the compiler (the matrix-core traces then run beside
-    // the products of the update).  Lanes that are not live carry D = 0 and add
-    // zeros; D stays finite because every ring position holds a link or zero.
-    if (d > 1) fetch(nxt, W);
+    // ring slot of ring offset o (relative to the start group): (o + rot) mod
+    // (W + 1); a sweep shifts it by one slot per start group, so that a slot
+    // keeps its slice from one start group to the next
     auto step = [&](int n, auto more_t) {
       constexpr bool more = decltype(more_t)::value;
       if constexpr (more) {
-        put((n + W) % (W + 1), nxt);
-        if (n + 2 < d) fetch(nxt, n + W + 1);
+        put((n + W + rot) % (W + 1), nxt);
+        // (sweep: at n = d - 2 the slice the next start group lacks; where no
+        // slice is needed the slice before is fetched again, from L2: in the
+        // sweep a fetch under a branch made the compiler wait for it within
+        // the step)
+        if constexpr (SWEEP)
+          fetch(nxt, (n + 2 < d || g + 1 < G) ? n + W + 1 : n + W);
+        else if (n + 2 < d)
+          fetch(nxt, n + W + 1);
       }
-      const char* slot = ring + ((n + w) % (W + 1)) * SLOT;
+      const char* slot = ring + ((n + w + rot) % (W + 1)) * SLOT;
       const v2<TS>* rs = (const v2<TS>*)slot + p;
       const v2<TC>* ru = (const v2<TC>*)(slot + NSP * (int)sizeof(v2<TS>)) + p;
       if constexpr (STORED) {
@@ -1120,19 +1084,136 @@ __global__ __launch_bounds__(64 * W, sizeof(TC) == 8 ? 2 : (STORED && W < 8 ? 3 
       }
       if constexpr (more) __syncthreads();
     };
-    for (int n = 0; n + 1 < d; n++) step(n, std::true_type());
-    step(d - 1, std::false_type());
-    __syncthreads();  // the ring of this block is read for the last time
-  }
-  __syncthreads();
-  for (int q = threadIdx.x; q < W * d * MM; q += NT) {
-    const int ww = q / (d * MM), n = (q / MM) % d, ij = q % MM;
-    const int xo = x0g + ww;
-    if (xo < N0)
-      partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] =
-          acc[(ww * d + n) * 16 + (ij / MSL) * 4 + ij % MSL];
+    if (!STORED && Ta == 1) {
+      fill(W + 1, [&](int j) { return slice_at(j - 1); }, [](int j) { return (j + W) % (W + 1); }, U0);
+    }
+    for (g = 0; g < (SWEEP ? G : 1); g++) {
+      if (g > 0) {
+        set_group(g);
+        form_next();
+      }
+
+      if constexpr (STORED) {
+        const long xslice = (long)(wave_live ? x0 : 0) * slice;
+  #pragma unroll
+        for (int m = 0; m < PL; m++) {
+          const int q = lane + 64 * m;
+          const int hh = q / 144, e = (q >> 4) % 9, pp = q & 15;
+          lok_[m] = q < NU && xs0 + pp < slice;
+          lo_[m] = lok_[m] ? (unsigned)((long)e * V + xslice + (hh ? xr_tab[pp] : xs0 + pp)) : 0u;
+        }
+        set_zero(Msrc);
+        if (live) {
+          M3<TS> t;
+          load(t, level_ptr(S, i), V, k, xslice + xs);
+          Msrc = convert<TC>(t);
+        }
+      }
+      set_zero(D);
+      if constexpr (STORED) {
+        // sinks of the slices x0g+Ta+o in ring slots o = 0 .. W-1; L_Ta into
+        // the buffer, L_{Ta+1} into registers
+        fill(W, slice_at, [](int j) { return j; }, U0);
+        lt_fetch(0);
+        lt_put();
+        if (d > 1) lt_fetch(1);
+        __syncthreads();
+      } else {
+      // the start slices x0g+w with S_i(x) and L_Ta at x and x + R k: for Ta = 1
+      // (L_1 = U_0) ring offsets -1 .. W-2, filled with offsets -1 .. W-1;
+      // otherwise slots 0 .. W-1, refilled with offsets 0 .. W-1 afterwards
+      // (Ta = 1: filled before the first start group; in a sweep the ring
+      // then holds the slices of every later one, the last of them stored at
+      // the end of the start group before)
+      if (Ta != 1) {
+        fill(W, [&](int j) { return (long)((x0g + j) % N0) * slice; }, [](int j) { return j; }, LTa);
+      }
+      __syncthreads();
+      // D_i(Ta) = L_Ta(x)^dag S_i(x) L_Ta(x + R k)
+      if (live) {
+        const char* slot = ring + (Ta == 1 ? (w + W + rot) % (W + 1) : w) * SLOT;
+        const v2<TS>* rs = (const v2<TS>*)slot + p;
+        const v2<TC>* ru = (const v2<TC>*)(slot + NSP * (int)sizeof(v2<TS>)) + p;
+        M3<TC> a;
+        M3<TS> t;
+  #pragma unroll
+        for (int e = 0; e < 9; e++) {
+          v2<TS> v = rs[i * SJ + e * 16];
+          t.re[e] = v.x;
+          t.im[e] = v.y;
+          v2<TC> c = ru[e * 16];
+          a.re[e] = c.x;
+          a.im[e] = c.y;
+        }
+        const M3<TC> h = dag_mul(a, convert<TC>(t));
+  #pragma unroll
+        for (int e = 0; e < 9; e++) {
+          v2<TC> c = ru[(9 + e) * 16];
+          a.re[e] = c.x;
+          a.im[e] = c.y;
+        }
+        D = mul(h, a);
+      }
+      __syncthreads();  // ring slots of the start slices are refilled next
+      if (Ta != 1) {
+        fill(W, slice_at, [](int j) { return j; }, U0);
+        __syncthreads();
+      }
+      }  // not STORED
+
+      // the slice of ring offset n+W (read by wave W-1 at step n+1) is fetched
+      // into registers during step n-1 and stored at the top of step n into the
+      // slot wave 0 read at step n-1, so a global load has a whole step to land.
+      // The last step is peeled (no refill, no update of D), so that the body of
+      // every step is one branch-free block: the traces and the update of D can
+      // be interleaved by the compiler (the matrix-core traces then run beside
+      // the products of the update).  Lanes that are not live carry D = 0 and add
+      // zeros; D stays finite because every ring position holds a link or zero.
+      if (d > 1) fetch(nxt, W);
+      for (int n = 0; n + 1 < d; n++) step(n, std::true_type());
+      step(d - 1, std::false_type());
+      __syncthreads();  // the ring of this start group is read for the last time; acc is complete
+      // (sweep) the slice the next start group lacks (ring offset 2W - 1 of
+      // this one, fetched at step d - 2) into the slot of offset W - 2, read
+      // for the last time at step W - 2
+      if (sweep && g + 1 < G) put((2 * W - 1 + rot) % (W + 1), nxt);
+      if (sweep) {
+        // this start group's sums into the workgroup's partial slots (the
+        // first pair block of the workgroup writes them), in pair-block order
+        // as without the sweep; each thread clears the entries it took
+        const bool first = bb == 3L * jw;
+        for (int q = threadIdx.x; q < W * d * MM; q += NT) {
+          const int ww = q / (d * MM), n = (q / MM) % d, ij = q % MM;
+          const int xo = x0g + ww;
+          const int a = (ww * d + n) * 16 + (ij / MSL) * 4 + ij % MSL;
+          if (xo < N0) {
+            double* dst = partial + (long)n * pstride + ((long)xo * nslot + jw) * MM + ij;
+            *dst = first ? acc[a] : *dst + acc[a];
+          }
+          acc[a] = 0.0;
+        }
+      }
+    }  // start groups
+  }  // pair blocks
+  if (sweep) {
+    // (sweep) a workgroup with no pair block: zero sums in its partial slots
+    if (3L * jw >= nb)
+      for (int q = threadIdx.x; q < N0 * d * MM; q += NT) {
+        const int xo = q / (d * MM), n = (q / MM) % d, ij = q % MM;
+        partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] = 0.0;
+      }
+  } else {
+    __syncthreads();
+    for (int q = threadIdx.x; q < W * d * MM; q += NT) {
+      const int ww = q / (d * MM), n = (q / MM) % d, ij = q % MM;
+      const int xo = x0g + ww;
+      if (xo < N0)
+        partial[(long)n * pstride + ((long)xo * nslot + jw) * MM + ij] =
+            acc[(ww * d + n) * 16 + (ij / MSL) * 4 + ij % MSL];
+    }
   }
 }
+
 
 // W_ij(R,T;x0) of one R from the partials of its (R,T) launches, stored in
 // wl_idx order.  partial layout: [t][x0][slot][i*msl+j] (nwg slots per x0).
@@ -1354,8 +1435,15 @@ template <int MSL, int W, class TS, class TC, bool STORED, bool MX>
 static void depth_launch(const Lat& L, int R, int Ta, int d, int nslot, const v2<TC>* LTa,
                          const v2<TC>* U0, const DepthArgs<TS>& a, hipStream_t st) {
   const size_t lds = contract_depth_lds(MSL, W, (int)sizeof(TS), (int)sizeof(TC), STORED, d);
+  if constexpr (!STORED && sizeof(TC) == 4) {
+    if (depth_sweep(STORED, true, Ta, d, W)) {
+      hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED, MX, true>), dim3((unsigned)nslot), dim3(64 * W),
+                         lds, st, L, R, Ta, d, nslot, LTa, U0, a.sp, a.fp, a.snext, a.partial, a.pstride);
+      return;
+    }
+  }
   const dim3 grid((unsigned)(((L.n[0] + W - 1) / W) * nslot));
-  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED, MX>), grid, dim3(64 * W), lds, st, L,
+  hipLaunchKernelGGL((contract_depth<MSL, W, TS, TC, STORED, MX, false>), grid, dim3(64 * W), lds, st, L,
                      R, Ta, d, nslot, LTa, U0, a.sp, a.fp, a.snext, a.partial, a.pstride);
 }
 
