@@ -684,6 +684,16 @@ __host__ __device__ constexpr bool depth_sweep(bool stored, bool fp32, int Ta, i
 // the same XCD when nslot is a multiple of 8.  Deterministic.  Requires
 // msl <= 4, d <= 16.
 //
+// SWEEP (FP32, T from 1, d = W; depth_sweep): the grid has nslot workgroups,
+// and workgroup b takes every start group of its pair blocks in turn.  Ring
+// offset o of start group g sits in slot (o + rot) mod (W + 1), rot = -g mod
+// (W + 1): a slot keeps its slice from one start group to the next, and the
+// ring at the end of a start group holds ring offsets -1 .. W - 2 of the next
+// but W - 1, which the last refill fetch of the start group brings in (stored
+// after its last step).  The sums of a start group are added into the
+// workgroup's partial slots (written by its first pair block) after its last
+// step, pair block after pair block as without the sweep.
+//
 // STORED (mixed path, T >= 10, or T >= 9 in form 8): the time transporters of
 // the block's T are the
 // stored FP32 copies of the FP64-accumulated L_T (LTa + (T-Ta) 9V), and
