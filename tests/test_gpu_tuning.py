@@ -281,3 +281,31 @@ def test_fused_forms_next_transporters(wl, mwlr, depth, lean, precision, kernel)
         print(f"[next S] kernel={kernel} prec={precision} mwlr={mwlr} depth={depth} lean={lean}", flush=True)
         w = lat.measure(p, precision)
     assert rel_err(w, ref) < (1e-12 if precision == 0 else 1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,wg", [((20, 4, 4, 4), 0), ((20, 6, 6, 6), 8)])
+@pytest.mark.parametrize("starts", [0, 4])
+def test_fused_sweep_start_groups(wl, dims, wg, starts):
+    """The FP32 T <= 8 launch sweeping the start groups (d = W = 8, the mixed
+    default) against the oracle and against the same launch without the sweep
+    (4 start slices per workgroup): 20x4^3 has 12 pair blocks for 512
+    workgroups (most own none and write zero sums); 20x6^3 with 8 workgroups
+    takes ~5 pair blocks per workgroup (the first writes the partial slots,
+    the others add); both over 3 start groups, the last ragged (20 = 8 + 8 +
+    4); open bc, T = 1..10 (T = 9 chain, T = 10 over the stored
+    transporters), R = 1, 2"""
+    nss, mwlt, mwlr = [0, 1, 3], 10, 2
+    u = O.generate(dims, 0, 51, 0.3)
+    ref, _ = O.ms10(dims, 0, u, (1.0, 1.0, 0.5), O.make_parms(nss, mwlt, mwlr))
+    p = wl.make_parms(nss, mwlt, mwlr)
+    with wl.Lattice(dims, 0) as lat:
+        lat.upload(u)
+        lat.tune(wl.TUNE_MIXED_CONTRACT_WG, wg)
+        lat.tune(wl.TUNE_MIXED_CONTRACT_STARTS, starts)
+        print(f"[sweep] dims={dims} wg={wg} starts={starts}", flush=True)
+        w = lat.measure(p, wl.MIXED)
+        again = lat.measure(p, wl.MIXED)
+    assert rel_err(w, ref) < 1e-6
+    assert np.array_equal(w, again)  # the partial slots are written afresh by every launch
+    assert np.array_equal(w == 0, ref == 0)
