@@ -27,6 +27,16 @@ __device__ __forceinline__ site_t stencil_site(int ny, int& y) {
   return (site_t)(b / ny) * 256 + threadIdx.x;  // every stencil launch has 256 threads
 }
 
+// Wave priority of the stencil kernels.  A thread first loads links and sums
+// staples (short products between loads), then combines and projects in FP64
+// (thousands of cycles with no memory access).  The staple phase runs at a
+// raised priority, so that a wave with loads to issue is not held behind waves
+// in their projections; the projections at the base priority then fill the
+// SIMD while the loads of the other waves are in flight (at 48^3x96: 7% off
+// the APE step, 3-4% off the HYP levels; the results are unchanged).
+__device__ __forceinline__ void staple_priority() { __builtin_amdgcn_s_setprio(2); }
+__device__ __forceinline__ void projection_priority() { __builtin_amdgcn_s_setprio(0); }
+
 // v += A_nu(x) B_mu(x+nu) A_nu(x+mu)^dag + A_nu(x-nu)^dag B_mu(x-nu) A_nu(x-nu+mu)
 // ln: slot of the nu-links in A, lm: slot of the mu-links in B.  PK: A and B
 // in the packed FP32 layout.
@@ -57,6 +67,7 @@ template <class TC, class TS, class TO = TS, bool PK = false>
 __device__ __forceinline__ void decorate_store(const M3<TC>& S, const v2<TS>* __restrict__ U, int lu,
                                                v2<TO>* __restrict__ out, int lo, long V, site_t s,
                                                double alpha, double beta, int proj, int piter) {
+  projection_priority();
   M3<TS> u;
   ld_link<PK>(u, U, V, lu, s);
   M3<double> w;
@@ -89,6 +100,7 @@ __global__ __launch_bounds__(256) void hyp_level1(Lat L, const v2<TS>* __restric
   int y;
   const site_t s = stencil_site(12, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = y / 3, eta = (mu + 1 + y % 3) & 3;
   int x[4];
   coords(L, s, x);
@@ -108,6 +120,7 @@ __global__ __launch_bounds__(256) void hyp_level2(Lat L, const v2<TS>* __restric
   int y;
   const site_t s = stencil_site(12, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = y / 3, nu = (mu + 1 + y % 3) & 3;
   int n = part4(mu, nu);
   int x[4];
@@ -132,6 +145,7 @@ __global__ __launch_bounds__(256) void hyp_level3(Lat L, const v2<TS>* U, v2<TS>
   int y;
   const site_t s = stencil_site(ny, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = mu0 + y;
   int x[4];
   coords(L, s, x);
@@ -164,6 +178,7 @@ __global__ __launch_bounds__(256) void hyp3_level1(Lat L, const v2<TS>* __restri
   int y;
   const site_t s = stencil_site(6, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = 1 + y / 2, eta = ((mu + y % 2) % 3) + 1;
   int x[4];
   coords(L, s, x);
@@ -184,6 +199,7 @@ __global__ __launch_bounds__(256) void hyp3_level2(Lat L, const v2<TS>* __restri
   int y;
   const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = 1 + y;
   int x[4];
   coords(L, s, x);
@@ -210,6 +226,7 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 4 : (sizeof(TC) == 4 ? 3 : 2
   int y;
   const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = 1 + y;
   int x[4];
   coords(L, s, x);
@@ -323,6 +340,7 @@ __global__ __launch_bounds__(256, sizeof(TS) == 4 ? 3 : 2) void ape_site(Lat L, 
                                                                       int piter) {
   const site_t s = block_site();
   if (s >= L.V) return;
+  staple_priority();
   int x[4];
   coords(L, s, x);
   M3<TC> S[3];
@@ -340,6 +358,7 @@ __global__ __launch_bounds__(256, sizeof(TC) == 4 ? 3 : 2) void ape_site_staples
                                                                               v2<TC>* __restrict__ S) {
   const site_t s = block_site();
   if (s >= L.V) return;
+  staple_priority();
   int x[4];
   coords(L, s, x);
   M3<TC> v[3];
@@ -359,6 +378,7 @@ __global__ __launch_bounds__(256, sizeof(TC) == 4 ? 4 : 2) void ape_staples(Lat 
   int y;
   const site_t s = stencil_site(3, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = 1 + y;
   int x[4];
   coords(L, s, x);
@@ -393,6 +413,7 @@ __global__ __launch_bounds__(256) void ape_step4(Lat L, const v2<TS>* __restrict
   int y;
   const site_t s = stencil_site(4, y);
   if (s >= L.V) return;
+  staple_priority();
   int mu = y;
   int x[4];
   coords(L, s, x);
